@@ -1,0 +1,180 @@
+/*
+ * orrery_lbm.h -- C ABI of the MI355X lattice Boltzmann library (liborrery_lbm.so).
+ *
+ * Drop-in boundary for the solver level of Orrery's aero LBM path: every entry point mirrors one
+ * member of orrery::aero::lbm::Lbm3DSolver / Lbm2DSolver or of the factory make_gpu_solver,
+ * which the reference binds to an IComputeBackend. Citations are reference-relative paths.
+ *
+ * Conventions
+ *  - Every call returns LBM_OK (0) or an error code; lbm_last_error() gives the message of the
+ *    last failing call on the calling thread. No C++ exception crosses this boundary.
+ *  - Host arrays are owned by the caller and are copied (or filled) before the call returns,
+ *    as IComputeBackend::upload/readback do (src/aero/compute/include/orrery/aero/compute/
+ *    compute_backend.h:89-93).
+ *  - Lattice layout: node (x, y, z) has index x + nx (y + ny z); populations are reported
+ *    direction-major, f[i * cells + cell], with the weights added back (the layout of
+ *    initial_populations, src/aero/lbm3d/include/orrery/aero/lbm3d/lbm3d.h:248-252).
+ *  - One handle binds one HIP device and one stream; one thread may call into it at a time.
+ */
+#ifndef ORRERY_LBM_H
+#define ORRERY_LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------ */
+enum {
+    LBM_OK = 0,
+    LBM_EINVAL = 1,  /* refused configuration or argument (validate(), check_binary32, ...) */
+    LBM_ENOMEM = 2,  /* refused before allocating: device memory (check_memory) */
+    LBM_EDEVICE = 3, /* a HIP runtime call failed */
+    LBM_ESTATE = 4   /* handle in a state that cannot serve the call */
+};
+
+/* ---- configuration (mirrors lbm3d.h:83-103, lbm2d.h:43-78) -------------------------------- */
+enum { LBM_X_PERIODIC = 0, LBM_X_INFLOW_OUTFLOW = 1 };           /* XBoundary, lbm2d.h:43 */
+enum { LBM_OUTLET_CONVECTIVE = 0, LBM_OUTLET_EXTRAPOLATION = 1 }; /* Outlet, lbm2d.h:50 */
+enum { LBM_WALL_PERIODIC = 0, LBM_WALL_SLIP = 1, LBM_WALL_NO_SLIP = 2 }; /* WallKind, lbm2d.h:51 */
+enum { LBM_SOLID_HALFWAY = 0, LBM_SOLID_INTERPOLATED = 1 };       /* SolidWall, lbm3d.h:161 */
+
+/* Population storage of the device lattice. The collision always runs in binary32 except for
+ * LBM_STORE_F64, the FP64 instantiation used as the on-device oracle. The 16-bit formats store
+ * the populations shifted by their lattice weights (f_i - w_i), as the reference's binary32
+ * buffers do (src/aero/lbm3d/lbm3d_gpu.cpp:1-7). */
+enum { LBM_STORE_F32 = 0, LBM_STORE_F16S = 1, LBM_STORE_BF16S = 2, LBM_STORE_F64 = 3 };
+
+typedef struct lbm_wall {
+    int32_t kind;      /* LBM_WALL_* */
+    double velocity_x; /* tangential wall speed along x, no-slip walls only (Wall, lbm2d.h:53-56) */
+} lbm_wall;
+
+typedef struct lbm3d_config { /* Lbm3DConfig, lbm3d.h:83-103 (+ Sponge :61-66) */
+    uint64_t nx, ny, nz;
+    double tau;
+    int32_t x_boundary; /* LBM_X_* */
+    int32_t outlet;     /* LBM_OUTLET_* */
+    lbm_wall y_low, y_high, z_low, z_high;
+    double inlet_ux, inlet_uy, inlet_uz;
+    double body_force_x, body_force_y, body_force_z;
+    int32_t smagorinsky_enabled;
+    double smagorinsky_constant;
+    double sponge_strength;
+    uint64_t sponge_inlet, sponge_outlet, sponge_sides;
+    int32_t solid_wall; /* LBM_SOLID_* */
+} lbm3d_config;
+
+typedef struct lbm2d_config { /* Lbm2DConfig, lbm2d.h:65-78 */
+    uint64_t nx, ny;
+    double tau;
+    int32_t x_boundary;
+    int32_t outlet;
+    lbm_wall bottom, top;
+    double inlet_ux, inlet_uy;
+    double body_force_x, body_force_y;
+    int32_t smagorinsky_enabled;
+    double smagorinsky_constant;
+} lbm2d_config;
+
+typedef struct lbm3d lbm3d; /* opaque solver handle (one device lattice) */
+typedef struct lbm2d lbm2d;
+
+/* ---- library ------------------------------------------------------------------------------ */
+/* Message of the last failing call on this thread ("" when none). */
+const char* lbm_last_error(void);
+/* Library version string. */
+const char* lbm_version(void);
+/* Free device memory and the largest single allocation, from hipMemGetInfo; replaces
+ * IComputeBackend::memory() (compute_backend.h:94-129). A fake limit set by
+ * lbm_set_memory_limit() caps `available` (the NullComputeBackend::set_memory analogue,
+ * src/aero/compute/include/orrery/aero/compute/null_compute_backend.h:22-86). */
+int lbm_device_memory(size_t* available, size_t* max_alloc);
+/* 0 clears the fake limit. */
+int lbm_set_memory_limit(size_t bytes);
+
+/* ---- D3Q19 -------------------------------------------------------------------------------- */
+/* make_gpu_solver (src/aero/lbm3d/lbm3d_gpu.cpp:487-513): validate() -> 32-bit index bound ->
+ * binary32 range -> boundary tables -> device-memory check -> allocate -> initialise with the
+ * uniform inlet state. Refuses before allocating, with the reference's reasons. `distance`
+ * (cells doubles) may be NULL unless solid_wall is interpolated. */
+int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double* distance,
+                 int32_t storage, lbm3d** out);
+/* ~GpuLbm3D: frees the device memory. NULL is accepted. */
+int lbm3d_destroy(lbm3d* h);
+/* Lbm3DSolver::initialize (lbm3d_gpu.cpp:298-331): equilibrium of the given state, solids at
+ * rest; resets the step count and the inlet velocity. `moments` holds 4 planes of `cells`
+ * doubles (rho, ux, uy, uz), or NULL for uniform_inlet_state (lbm3d_common.cpp:262-270). */
+int lbm3d_initialize(lbm3d* h, const double* moments);
+/* initialize() with density 1 and the inlet velocity (rest when x is periodic) plus a seeded
+ * uniform perturbation of +-amplitude on each velocity component of every fluid node, drawn
+ * from lbm_perturbation(seed, cell, component). The state is built on the device. */
+int lbm3d_initialize_perturbed(lbm3d* h, uint64_t seed, double amplitude);
+/* Lbm3DSolver::step (lbm3d_gpu.cpp:333-355): advances exactly `count` steps. */
+int lbm3d_step(lbm3d* h, uint64_t count);
+/* Lbm3DSolver::steps. */
+uint64_t lbm3d_steps(const lbm3d* h);
+/* set_inlet_velocity / inlet_velocity (lbm3d_gpu.cpp:268-289, check lbm3d_common.cpp:163-174). */
+int lbm3d_set_inlet(lbm3d* h, const double velocity[3]);
+int lbm3d_get_inlet(const lbm3d* h, double velocity[3]);
+/* Lbm3DSolver::force (lbm3d_cpu.cpp:192-203, lbm3d_common.cpp:213-241): momentum exchange
+ * over the boundary links, moment about the lattice origin at the wall points; summed in
+ * FP64 on the device in a fixed order (deterministic). */
+int lbm3d_force(lbm3d* h, double force[3], double moment[3]);
+/* Lbm3DSolver::mass (lbm3d_cpu.cpp:205-215): sum of the populations over fluid nodes, FP64. */
+int lbm3d_mass(lbm3d* h, double* mass);
+/* Lbm3DSolver::macroscopic (lbm3d_cpu.cpp:154-186): density and velocity (Guo half-force
+ * corrected) per node; solids report 1 and 0. Any pointer may be NULL. */
+int lbm3d_macroscopic(lbm3d* h, double* rho, double* ux, double* uy, double* uz);
+/* Lbm3DSolver::populations (lbm3d_gpu.cpp:394-405): 19 * cells doubles, logical layout. */
+int lbm3d_populations(lbm3d* h, double* f);
+/* Lbm3DSolver::links (boundary_links, lbm3d_common.cpp:176-198): writes up to `capacity`
+ * links and sets *count to the total; any output pointer may be NULL. */
+int lbm3d_links(const lbm3d* h, uint32_t* cell, uint32_t* direction, double* q, size_t capacity,
+                size_t* count);
+/* Device bytes held by the handle (populations, node classes, boundary tables, history). */
+int lbm3d_device_bytes(const lbm3d* h, size_t* bytes);
+/* Blocks until the handle's stream has drained (the reference's finish()). */
+int lbm3d_synchronize(lbm3d* h);
+/* Launch shape of the step kernel: block size (64..256, multiple of 64) and cells per
+ * thread (1, 2 or 4). 0 keeps the current value. Used by the autotuner. */
+int lbm3d_set_launch(lbm3d* h, int32_t block, int32_t cells_per_thread);
+/* Bytes one step of this lattice moves by the algorithmic count: 2 * 19 * bytes per stored
+ * population, times nx * ny * nz (the reference's MLUPS convention). */
+int lbm3d_step_bytes(const lbm3d* h, double* bytes);
+/* Node counts of the lattice: bulk (plain streaming), boundary, solid, and boundary links. */
+int lbm3d_counts(const lbm3d* h, size_t* bulk, size_t* boundary, size_t* solid, size_t* links);
+/* Advances `count` steps and times them with HIP events on the handle's stream (the
+ * reference's bench protocol, tests/bench/aero/bench_lbm_gpu.cpp:54-64). With per_kernel = 0,
+ * one event pair brackets all steps (*step_ms); with per_kernel = 1, an event pair also
+ * brackets every launch, giving the summed durations of the bulk and boundary kernels. */
+int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_ms, double* bulk_ms,
+                     double* boundary_ms);
+
+/* ---- D2Q9 (same shape; lbm2d.h:173-211, lbm2d_gpu.cpp:68-378) ---------------------------- */
+int lbm2d_create(const lbm2d_config* config, const uint8_t* solid, int32_t storage, lbm2d** out);
+int lbm2d_destroy(lbm2d* h);
+int lbm2d_initialize(lbm2d* h, const double* moments /* 3 planes: rho, ux, uy, or NULL */);
+int lbm2d_initialize_perturbed(lbm2d* h, uint64_t seed, double amplitude);
+int lbm2d_step(lbm2d* h, uint64_t count);
+uint64_t lbm2d_steps(const lbm2d* h);
+int lbm2d_force(lbm2d* h, double* fx, double* fy, double* moment);
+int lbm2d_mass(lbm2d* h, double* mass);
+int lbm2d_macroscopic(lbm2d* h, double* rho, double* ux, double* uy);
+int lbm2d_populations(lbm2d* h, double* f);
+int lbm2d_links(const lbm2d* h, uint32_t* cell, uint32_t* direction, size_t capacity,
+                size_t* count);
+int lbm2d_synchronize(lbm2d* h);
+
+/* ---- shared helpers ----------------------------------------------------------------------- */
+/* Seeded perturbation in [-1, 1): SplitMix64 of (seed, cell, component), 53-bit mantissa.
+ * The same counter-based draw on the host and the device, so every solver starts alike. */
+double lbm_perturbation(uint64_t seed, uint64_t cell, uint32_t component);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORRERY_LBM_H */

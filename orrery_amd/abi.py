@@ -1,0 +1,123 @@
+"""ctypes mirror of include/orrery_lbm.h: the configuration structs and the library loader.
+
+The structs follow Lbm3DConfig / Lbm2DConfig of the reference
+(src/aero/lbm3d/include/orrery/aero/lbm3d/lbm3d.h:83-103,
+src/aero/lbm/include/orrery/aero/lbm/lbm2d.h:43-78).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "liborrery_lbm.so")
+
+LBM_OK, LBM_EINVAL, LBM_ENOMEM, LBM_EDEVICE, LBM_ESTATE = 0, 1, 2, 3, 4
+STORAGE = {"f32": 0, "f16s": 1, "bf16s": 2, "f64": 3}
+STORAGE_BYTES = {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}
+
+
+class CWall(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("velocity_x", ctypes.c_double)]
+
+
+class CConfig3D(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint64), ("ny", ctypes.c_uint64), ("nz", ctypes.c_uint64),
+        ("tau", ctypes.c_double),
+        ("x_boundary", ctypes.c_int32), ("outlet", ctypes.c_int32),
+        ("y_low", CWall), ("y_high", CWall), ("z_low", CWall), ("z_high", CWall),
+        ("inlet_ux", ctypes.c_double), ("inlet_uy", ctypes.c_double), ("inlet_uz", ctypes.c_double),
+        ("body_force_x", ctypes.c_double), ("body_force_y", ctypes.c_double),
+        ("body_force_z", ctypes.c_double),
+        ("smagorinsky_enabled", ctypes.c_int32), ("smagorinsky_constant", ctypes.c_double),
+        ("sponge_strength", ctypes.c_double),
+        ("sponge_inlet", ctypes.c_uint64), ("sponge_outlet", ctypes.c_uint64),
+        ("sponge_sides", ctypes.c_uint64),
+        ("solid_wall", ctypes.c_int32),
+    ]
+
+
+class CConfig2D(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint64), ("ny", ctypes.c_uint64),
+        ("tau", ctypes.c_double),
+        ("x_boundary", ctypes.c_int32), ("outlet", ctypes.c_int32),
+        ("bottom", CWall), ("top", CWall),
+        ("inlet_ux", ctypes.c_double), ("inlet_uy", ctypes.c_double),
+        ("body_force_x", ctypes.c_double), ("body_force_y", ctypes.c_double),
+        ("smagorinsky_enabled", ctypes.c_int32), ("smagorinsky_constant", ctypes.c_double),
+    ]
+
+
+P = ctypes.POINTER
+c_u8p = P(ctypes.c_uint8)
+c_u32p = P(ctypes.c_uint32)
+c_f64p = P(ctypes.c_double)
+c_sizep = P(ctypes.c_size_t)
+
+# (name, restype, argtypes) of every export declared in include/orrery_lbm.h.
+EXPORTS = [
+    ("lbm_last_error", ctypes.c_char_p, []),
+    ("lbm_version", ctypes.c_char_p, []),
+    ("lbm_device_memory", ctypes.c_int, [c_sizep, c_sizep]),
+    ("lbm_set_memory_limit", ctypes.c_int, [ctypes.c_size_t]),
+    ("lbm3d_create", ctypes.c_int, [P(CConfig3D), c_u8p, c_f64p, ctypes.c_int32, P(ctypes.c_void_p)]),
+    ("lbm3d_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm3d_initialize", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm3d_initialize_perturbed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double]),
+    ("lbm3d_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    ("lbm3d_steps", ctypes.c_uint64, [ctypes.c_void_p]),
+    ("lbm3d_set_inlet", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm3d_get_inlet", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm3d_force", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p]),
+    ("lbm3d_mass", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm3d_macroscopic", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p, c_f64p]),
+    ("lbm3d_populations", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm3d_links", ctypes.c_int, [ctypes.c_void_p, c_u32p, c_u32p, c_f64p, ctypes.c_size_t, c_sizep]),
+    ("lbm3d_device_bytes", ctypes.c_int, [ctypes.c_void_p, c_sizep]),
+    ("lbm3d_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm3d_set_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    ("lbm3d_step_bytes", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm3d_counts", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep, c_sizep, c_sizep]),
+    ("lbm3d_time_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, c_f64p, c_f64p, c_f64p]),
+    ("lbm2d_create", ctypes.c_int, [P(CConfig2D), c_u8p, ctypes.c_int32, P(ctypes.c_void_p)]),
+    ("lbm2d_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm2d_initialize", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm2d_initialize_perturbed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double]),
+    ("lbm2d_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    ("lbm2d_steps", ctypes.c_uint64, [ctypes.c_void_p]),
+    ("lbm2d_force", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p]),
+    ("lbm2d_mass", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm2d_macroscopic", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p]),
+    ("lbm2d_populations", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm2d_links", ctypes.c_int, [ctypes.c_void_p, c_u32p, c_u32p, ctypes.c_size_t, c_sizep]),
+    ("lbm2d_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm_perturbation", ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]),
+]
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads liborrery_lbm.so (built in-tree by __graft_entry__.build()); raises when missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY):
+        raise RuntimeError(
+            f"orrery_amd: the HIP library {LIBRARY} is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIBRARY)
+    for name, restype, argtypes in EXPORTS:
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def ptr(array, ctype):
+    """Pointer to the data of a contiguous numpy array (None passes NULL)."""
+    if array is None:
+        return None
+    return array.ctypes.data_as(P(ctype))

@@ -1,0 +1,45 @@
+"""Build recipe of liborrery_lbm.so (HIP, gfx950), compiled in-tree next to this file.
+
+    python -m orrery_amd.build
+
+Every HIP source is compiled for gfx950 without floating-point contraction (the binary64
+kernels reproduce the CPU solver's rounding) and without packed-FP32 instructions.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+SOURCES = ["lbm_setup.hip", "lbm3d_kernels.hip", "lbm3d_abi.hip", "lbm2d_kernels.hip", "lbm2d_abi.hip",
+           "lbm_batch.hip"]
+OUTPUT = os.path.join(HERE, "liborrery_lbm.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+]
+
+
+def sources() -> list[str]:
+    return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+
+
+def build(verbose: bool = False, extra: list[str] | None = None) -> str:
+    """Compiles the library; returns its path. Raises CalledProcessError on failure."""
+    tmp = OUTPUT + ".tmp"
+    cmd = [HIPCC, *FLAGS, *(extra or []), "-o", tmp, *sources()]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, OUTPUT)
+    return OUTPUT
+
+
+if __name__ == "__main__":
+    build(verbose=True, extra=sys.argv[1:])

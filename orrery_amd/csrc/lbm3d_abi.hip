@@ -1,0 +1,706 @@
+// C ABI of the D3Q19 device solver (include/orrery_lbm.h): the handle that replaces GpuLbm3D
+// and make_gpu_solver (src/aero/lbm3d/lbm3d_gpu.cpp:151-513).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lbm_device.h"
+#include "lbm_host.h"
+#include "lbm_kernels.h"
+
+using lbm::host::fail;
+using lbm::host::ok;
+
+namespace {
+
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    std::size_t bytes = 0;
+};
+
+} // namespace
+
+struct lbm3d {
+    lbm3d_config cfg{};
+    int storage = LBM_STORE_F32;
+    std::size_t cells = 0;
+    std::uint32_t stride = 0;
+    lbm::host::Tables tables;
+    std::vector<std::uint8_t> solid;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::uint64_t steps = 0;
+    double inlet[3] = {0, 0, 0};
+    int block = 256;
+    int cpt = 1;
+    // device memory
+    void* f = nullptr;
+    std::uint8_t* cls = nullptr;
+    std::uint32_t* bcell = nullptr;
+    std::uint32_t* bcode = nullptr;
+    void* bq = nullptr;
+    void* stash = nullptr;
+    void* hist = nullptr;
+    std::uint32_t* lcell = nullptr;
+    std::uint8_t* ldir = nullptr;
+    double* lq = nullptr;
+    std::uint32_t* lslot = nullptr;
+    double* partial = nullptr;
+    double* result = nullptr;
+    std::size_t device_bytes = 0;
+    std::vector<void*> owned;
+};
+
+namespace {
+
+std::size_t store_bytes(int storage) {
+    switch (storage) {
+        case LBM_STORE_F16S:
+        case LBM_STORE_BF16S:
+            return 2;
+        case LBM_STORE_F64:
+            return 8;
+        default:
+            return 4;
+    }
+}
+
+std::size_t real_bytes(int storage) { return storage == LBM_STORE_F64 ? 8 : 4; }
+
+int hip_fail(const char* what, hipError_t e) {
+    return fail(LBM_EDEVICE, std::string("lbm3d gpu: ") + what + ": " + hipGetErrorString(e));
+}
+
+// Device bytes of a lattice with these tables (the admission rule of the serving loop and of
+// the memory check; the counterpart of gpu_memory_bytes, lbm3d_common.cpp:276-300).
+std::size_t lattice_bytes(std::size_t cells, std::uint32_t stride, const lbm::host::Tables& t, int storage,
+                          bool interpolated) {
+    const std::size_t sb = store_bytes(storage);
+    const std::size_t rb = real_bytes(storage);
+    std::size_t total = 19U * static_cast<std::size_t>(stride) * sb; // populations (in place)
+    total += cells;                                                     // node classes (u8)
+    total += t.bcell.size() * 4U;                                       // boundary list
+    total += 19U * static_cast<std::size_t>(t.bstride) * 4U;            // source codes
+    if (interpolated) {
+        total += 19U * static_cast<std::size_t>(t.bstride) * rb; // wall fractions
+    }
+    total += 2U * 19U * static_cast<std::size_t>(t.bstride) * rb; // stash
+    total += 5U * static_cast<std::size_t>(t.bstride) * rb;       // outlet history
+    total += t.links.size() * (4U + 1U + 8U + 4U);                 // links
+    const std::size_t red = std::max<std::size_t>(lbm::reduction_blocks(cells) + lbm::reduction_blocks(t.bcell.size()),
+                                                  6U * lbm::reduction_blocks(t.links.size()));
+    total += (red + 8U) * 8U;
+    return total;
+}
+
+template <class T> lbm::Lattice3<typename T::R> descriptor(const lbm3d& h) {
+    using R = typename T::R;
+    lbm::Lattice3<R> p{};
+    const lbm3d_config& c = h.cfg;
+    p.f = h.f;
+    p.cls = h.cls;
+    p.bcell = h.bcell;
+    p.bcode = h.bcode;
+    p.bq = static_cast<const R*>(h.bq);
+    p.stash = static_cast<R*>(h.stash);
+    p.hist = static_cast<R*>(h.hist);
+    p.nx = static_cast<std::uint32_t>(c.nx);
+    p.ny = static_cast<std::uint32_t>(c.ny);
+    p.nz = static_cast<std::uint32_t>(c.nz);
+    p.cells = static_cast<std::uint32_t>(h.cells);
+    p.stride = h.stride;
+    p.nb = static_cast<std::uint32_t>(h.tables.bcell.size());
+    p.bstride = h.tables.bstride;
+    p.div_nx = lbm::FastDiv(p.nx);
+    p.div_plane = lbm::FastDiv(p.nx * p.ny);
+    p.x_mode = c.x_boundary;
+    p.convective = c.outlet == LBM_OUTLET_CONVECTIVE ? 1 : 0;
+    p.side[0] = c.y_low.kind;
+    p.side[1] = c.y_high.kind;
+    p.side[2] = c.z_low.kind;
+    p.side[3] = c.z_high.kind;
+    p.wall_u[0] = static_cast<R>(c.y_low.velocity_x);
+    p.wall_u[1] = static_cast<R>(c.y_high.velocity_x);
+    p.wall_u[2] = static_cast<R>(c.z_low.velocity_x);
+    p.wall_u[3] = static_cast<R>(c.z_high.velocity_x);
+    for (int a = 0; a < 3; ++a) {
+        p.inlet[a] = static_cast<R>(h.inlet[a]);
+    }
+    p.tau = static_cast<R>(c.tau);
+    p.omega = static_cast<R>(1) / static_cast<R>(c.tau);
+    const double k = c.smagorinsky_enabled ? c.smagorinsky_constant : 0.0;
+    p.smag_c = static_cast<R>(k);
+    p.smag_c2 = static_cast<R>(k * k);
+    // The binary32 kernels switch the closure off when C^2 is 0, as the GLSL does
+    // (lbm_d3q19_common.glsl:287); the binary64 one follows the CPU's flag.
+    p.les = T::shifted ? (p.smag_c2 > R(0) ? 1 : 0) : (c.smagorinsky_enabled ? 1 : 0);
+    p.force[0] = static_cast<R>(c.body_force_x);
+    p.force[1] = static_cast<R>(c.body_force_y);
+    p.force[2] = static_cast<R>(c.body_force_z);
+    p.forced = (c.body_force_x != 0.0 || c.body_force_y != 0.0 || c.body_force_z != 0.0) ? 1 : 0;
+    p.outlet_speed = static_cast<R>(c.inlet_ux);
+    const bool sponge = c.sponge_strength > 0.0 && c.x_boundary == LBM_X_INFLOW_OUTFLOW;
+    p.sponge_strength = sponge ? static_cast<R>(c.sponge_strength) : R(0);
+    p.sponge_in = static_cast<std::uint32_t>(c.sponge_inlet);
+    p.sponge_out = static_cast<std::uint32_t>(c.sponge_outlet);
+    p.sponge_sides = static_cast<std::uint32_t>(c.sponge_sides);
+    p.sponge_y = c.y_low.kind != LBM_WALL_PERIODIC ? 1 : 0;
+    p.sponge_z = c.z_low.kind != LBM_WALL_PERIODIC ? 1 : 0;
+    return p;
+}
+
+// Runs `fn.template operator()<T>()` for the storage format of the handle.
+template <class Fn> int with_store(const lbm3d& h, Fn&& fn) {
+    switch (h.storage) {
+        case LBM_STORE_F16S:
+            return fn.template operator()<lbm::StoreF16>();
+        case LBM_STORE_BF16S:
+            return fn.template operator()<lbm::StoreBF16>();
+        case LBM_STORE_F64:
+            return fn.template operator()<lbm::StoreF64>();
+        default:
+            return fn.template operator()<lbm::StoreF32>();
+    }
+}
+
+int alloc(lbm3d& h, void** out, std::size_t bytes, const char* what) {
+    *out = nullptr;
+    if (bytes == 0) {
+        return ok();
+    }
+    const hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(LBM_ENOMEM, std::string("lbm3d gpu: ") + what + ": " + hipGetErrorString(e));
+    }
+    h.owned.push_back(*out);
+    h.device_bytes += bytes;
+    return ok();
+}
+
+template <typename V> int upload(lbm3d& h, void* dst, const std::vector<V>& src, const char* what) {
+    if (src.empty()) {
+        return ok();
+    }
+    const hipError_t e = hipMemcpyAsync(dst, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice, h.stream);
+    return e == hipSuccess ? ok() : hip_fail(what, e);
+}
+
+void release(lbm3d* h) {
+    if (h == nullptr) {
+        return;
+    }
+    if (h->stream != nullptr) {
+        (void)hipStreamSynchronize(h->stream);
+    }
+    for (void* p : h->owned) {
+        (void)hipFree(p);
+    }
+    if (h->stream != nullptr) {
+        (void)hipStreamDestroy(h->stream);
+    }
+    delete h;
+}
+
+int sync(lbm3d& h, const char* what) {
+    const hipError_t e = hipStreamSynchronize(h.stream);
+    return e == hipSuccess ? ok() : hip_fail(what, e);
+}
+
+int initialize_mode(lbm3d* h, int mode, const double* host_planes, std::uint64_t seed, double amplitude) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    const bool inflow = h->cfg.x_boundary == LBM_X_INFLOW_OUTFLOW;
+    const double u0[3] = {inflow ? h->cfg.inlet_ux : 0.0, inflow ? h->cfg.inlet_uy : 0.0,
+                          inflow ? h->cfg.inlet_uz : 0.0};
+    double* planes = nullptr;
+    if (mode == 1) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&planes), 4U * h->cells * sizeof(double));
+        if (e != hipSuccess) {
+            return fail(LBM_ENOMEM, std::string("lbm3d gpu: initial state buffer: ") + hipGetErrorString(e));
+        }
+        e = hipMemcpyAsync(planes, host_planes, 4U * h->cells * sizeof(double), hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(planes);
+            return hip_fail("initial state upload", e);
+        }
+    }
+    const int rc = with_store(*h, [&]<class T>() -> int {
+        const hipError_t e = lbm::d3q19_initialize<T>(descriptor<T>(*h), mode, planes, u0, seed, amplitude, h->stream);
+        return e == hipSuccess ? ok() : hip_fail("initialize", e);
+    });
+    const int done = sync(*h, "initialize");
+    if (planes != nullptr) {
+        (void)hipFree(planes);
+    }
+    if (rc != LBM_OK) {
+        return rc;
+    }
+    if (done != LBM_OK) {
+        return done;
+    }
+    h->steps = 0;
+    h->inlet[0] = h->cfg.inlet_ux;
+    h->inlet[1] = h->cfg.inlet_uy;
+    h->inlet[2] = h->cfg.inlet_uz;
+    return ok();
+}
+
+} // namespace
+
+extern "C" {
+
+int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double* distance, int32_t storage,
+                 lbm3d** out) {
+    if (out == nullptr || config == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    *out = nullptr;
+    if (storage < LBM_STORE_F32 || storage > LBM_STORE_F64) {
+        return fail(LBM_EINVAL, "lbm3d gpu: unknown population storage");
+    }
+    const lbm3d_config& c = *config;
+    // validate() (lbm3d_common.cpp:159-161)
+    if (std::string p = lbm::host::problem3(c, solid, distance); !p.empty()) {
+        return fail(LBM_EINVAL, p);
+    }
+    const std::size_t cells = c.nx * c.ny * c.nz;
+    const std::uint32_t stride = lbm::host::padded(cells, 128);
+    // Every index the kernels form (population i * stride + cell) must fit in 32 bits
+    // (lbm3d_gpu.cpp:492-497).
+    if (static_cast<std::uint64_t>(stride) * 19U > std::numeric_limits<std::uint32_t>::max()) {
+        return fail(LBM_EINVAL, "lbm3d gpu: lattice too large for 32-bit indices");
+    }
+    if (storage != LBM_STORE_F64) {
+        if (std::string p = lbm::host::binary32_problem3(c); !p.empty()) {
+            return fail(LBM_EINVAL, p);
+        }
+    }
+    auto* h = new (std::nothrow) lbm3d();
+    if (h == nullptr) {
+        return fail(LBM_ENOMEM, "lbm3d: host allocation failed");
+    }
+    h->cfg = c;
+    h->storage = storage;
+    h->cells = cells;
+    h->stride = stride;
+    h->solid.assign(solid, solid + cells);
+    h->tables = lbm::host::tables3(c, solid, distance);
+    const bool interpolated = c.solid_wall == LBM_SOLID_INTERPOLATED;
+    // check_memory (lbm3d_gpu.cpp:131-149): refuse before allocating.
+    const std::size_t need = lattice_bytes(cells, stride, h->tables, storage, interpolated);
+    std::size_t available = 0;
+    if (int rc = lbm::host::device_memory(&available, nullptr); rc != LBM_OK) {
+        delete h;
+        return rc;
+    }
+    if (need > available) {
+        delete h;
+        return fail(LBM_ENOMEM, "lbm3d gpu: the lattice needs " + lbm::host::megabytes(need) +
+                                    " of device memory, " + lbm::host::megabytes(available) +
+                                    " are available; use fewer cells");
+    }
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess) {
+        e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    }
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail("stream", e);
+    }
+    const lbm::host::Tables& t = h->tables;
+    const std::size_t sb = store_bytes(storage);
+    const std::size_t rb = real_bytes(storage);
+    const std::size_t nb = t.bcell.size();
+    const std::size_t nl = t.links.size();
+    const std::size_t red = std::max<std::size_t>(lbm::reduction_blocks(cells) + lbm::reduction_blocks(nb),
+                                                  6U * lbm::reduction_blocks(nl));
+    int rc = LBM_OK;
+    void* tmp = nullptr;
+    if ((rc = alloc(*h, &h->f, 19U * static_cast<std::size_t>(stride) * sb, "population buffer")) ||
+        (rc = alloc(*h, &tmp, cells, "node class buffer")) || ((h->cls = static_cast<std::uint8_t*>(tmp)), false) ||
+        (rc = alloc(*h, &tmp, std::max<std::size_t>(nb, 1) * 4U, "boundary list")) ||
+        ((h->bcell = static_cast<std::uint32_t*>(tmp)), false) ||
+        (rc = alloc(*h, &tmp, 19U * static_cast<std::size_t>(t.bstride) * 4U, "source codes")) ||
+        ((h->bcode = static_cast<std::uint32_t*>(tmp)), false) ||
+        (interpolated && (rc = alloc(*h, &h->bq, 19U * static_cast<std::size_t>(t.bstride) * rb, "wall buffer"))) ||
+        (rc = alloc(*h, &h->stash, 2U * 19U * static_cast<std::size_t>(t.bstride) * rb, "boundary stash")) ||
+        (rc = alloc(*h, &h->hist, 5U * static_cast<std::size_t>(t.bstride) * rb, "outlet buffer")) ||
+        (rc = alloc(*h, &tmp, std::max<std::size_t>(nl, 1) * 4U, "link buffer")) ||
+        ((h->lcell = static_cast<std::uint32_t*>(tmp)), false) ||
+        (rc = alloc(*h, &tmp, std::max<std::size_t>(nl, 1), "link buffer")) ||
+        ((h->ldir = static_cast<std::uint8_t*>(tmp)), false) ||
+        (rc = alloc(*h, &tmp, std::max<std::size_t>(nl, 1) * 8U, "link buffer")) ||
+        ((h->lq = static_cast<double*>(tmp)), false) ||
+        (rc = alloc(*h, &tmp, std::max<std::size_t>(nl, 1) * 4U, "link buffer")) ||
+        ((h->lslot = static_cast<std::uint32_t*>(tmp)), false) ||
+        (rc = alloc(*h, &tmp, (red + 8U) * 8U, "reduction buffer")) || ((h->partial = static_cast<double*>(tmp)), false)) {
+        release(h);
+        return rc;
+    }
+    h->result = h->partial + red;
+    // Upload the tables.
+    std::vector<std::uint32_t> lcell(nl);
+    std::vector<std::uint8_t> ldir(nl);
+    std::vector<double> lq(nl);
+    for (std::size_t k = 0; k < nl; ++k) {
+        lcell[k] = t.links[k].cell;
+        ldir[k] = static_cast<std::uint8_t>(t.links[k].direction);
+        lq[k] = t.links[k].q;
+    }
+    if ((rc = upload(*h, h->cls, t.cls, "node class upload")) || (rc = upload(*h, h->bcell, t.bcell, "boundary upload")) ||
+        (rc = upload(*h, h->bcode, t.bcode, "code upload")) || (rc = upload(*h, h->lcell, lcell, "link upload")) ||
+        (rc = upload(*h, h->ldir, ldir, "link upload")) || (rc = upload(*h, h->lq, lq, "link upload")) ||
+        (rc = upload(*h, h->lslot, t.lslot, "link upload"))) {
+        release(h);
+        return rc;
+    }
+    if (interpolated) {
+        if (storage == LBM_STORE_F64) {
+            rc = upload(*h, h->bq, t.bq, "wall upload");
+        } else {
+            std::vector<float> q32(t.bq.begin(), t.bq.end());
+            rc = upload(*h, h->bq, q32, "wall upload");
+        }
+        if (rc != LBM_OK) {
+            release(h);
+            return rc;
+        }
+    }
+    // Zero the history and the stash so that no stale bits are ever read.
+    if ((e = hipMemsetAsync(h->hist, 0, 5U * static_cast<std::size_t>(t.bstride) * rb, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(h->stash, 0, 2U * 19U * static_cast<std::size_t>(t.bstride) * rb, h->stream)) !=
+            hipSuccess) {
+        release(h);
+        return hip_fail("clear", e);
+    }
+    if ((rc = initialize_mode(h, 0, nullptr, 0, 0.0)) != LBM_OK) {
+        release(h);
+        return rc;
+    }
+    *out = h;
+    return ok();
+}
+
+int lbm3d_destroy(lbm3d* h) {
+    release(h);
+    return ok();
+}
+
+int lbm3d_initialize(lbm3d* h, const double* moments) {
+    return initialize_mode(h, moments != nullptr ? 1 : 0, moments, 0, 0.0);
+}
+
+int lbm3d_initialize_perturbed(lbm3d* h, uint64_t seed, double amplitude) {
+    if (!std::isfinite(amplitude)) {
+        return fail(LBM_EINVAL, "lbm3d: the perturbation amplitude must be finite");
+    }
+    return initialize_mode(h, 2, nullptr, seed, amplitude);
+}
+
+int lbm3d_step(lbm3d* h, uint64_t count) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    for (std::uint64_t n = 0; n < count; ++n) {
+        const bool local = (h->steps & 1U) != 0;
+        const int rc = with_store(*h, [&]<class T>() -> int {
+            const hipError_t e = lbm::d3q19_step<T>(descriptor<T>(*h), local, h->block, h->cpt, h->stream);
+            return e == hipSuccess ? ok() : hip_fail("step", e);
+        });
+        if (rc != LBM_OK) {
+            return rc;
+        }
+        ++h->steps;
+    }
+    return ok();
+}
+
+uint64_t lbm3d_steps(const lbm3d* h) { return h != nullptr ? h->steps : 0; }
+
+int lbm3d_set_inlet(lbm3d* h, const double velocity[3]) {
+    if (h == nullptr || velocity == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    if (std::string p = lbm::host::inlet_problem3(h->cfg, velocity); !p.empty()) {
+        return fail(LBM_EINVAL, p);
+    }
+    for (int a = 0; a < 3; ++a) {
+        h->inlet[a] = velocity[a];
+    }
+    return ok();
+}
+
+int lbm3d_get_inlet(const lbm3d* h, double velocity[3]) {
+    if (h == nullptr || velocity == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    for (int a = 0; a < 3; ++a) {
+        velocity[a] = h->inlet[a];
+    }
+    return ok();
+}
+
+int lbm3d_force(lbm3d* h, double force[3], double moment[3]) {
+    if (h == nullptr || force == nullptr || moment == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    const std::size_t nl = h->tables.links.size();
+    if (nl == 0) {
+        for (int a = 0; a < 3; ++a) {
+            force[a] = 0.0;
+            moment[a] = 0.0;
+        }
+        return ok();
+    }
+    const int phase = static_cast<int>(h->steps & 1U);
+    int rc = with_store(*h, [&]<class T>() -> int {
+        const hipError_t e = lbm::d3q19_force_sum<T>(descriptor<T>(*h), phase, h->lcell, h->ldir, h->lq, h->lslot,
+                                                     static_cast<std::uint32_t>(nl), h->partial, h->result, h->stream);
+        return e == hipSuccess ? ok() : hip_fail("force", e);
+    });
+    if (rc != LBM_OK) {
+        return rc;
+    }
+    double v[6];
+    hipError_t e = hipMemcpyAsync(v, h->result, sizeof(v), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) {
+        e = hipStreamSynchronize(h->stream);
+    }
+    if (e != hipSuccess) {
+        return hip_fail("force readback", e);
+    }
+    for (int a = 0; a < 3; ++a) {
+        force[a] = v[a];
+        moment[a] = v[3 + a];
+    }
+    return ok();
+}
+
+int lbm3d_mass(lbm3d* h, double* mass) {
+    if (h == nullptr || mass == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    const int phase = static_cast<int>(h->steps & 1U);
+    int rc = with_store(*h, [&]<class T>() -> int {
+        const hipError_t e = lbm::d3q19_mass<T>(descriptor<T>(*h), phase, h->partial, h->result, h->stream);
+        return e == hipSuccess ? ok() : hip_fail("mass", e);
+    });
+    if (rc != LBM_OK) {
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(mass, h->result, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) {
+        e = hipStreamSynchronize(h->stream);
+    }
+    return e == hipSuccess ? ok() : hip_fail("mass readback", e);
+}
+
+int lbm3d_macroscopic(lbm3d* h, double* rho, double* ux, double* uy, double* uz) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    double* planes = nullptr;
+    const std::size_t bytes = 4U * h->cells * sizeof(double);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&planes), bytes);
+    if (e != hipSuccess) {
+        return fail(LBM_ENOMEM, std::string("lbm3d gpu: moment buffer: ") + hipGetErrorString(e));
+    }
+    const int phase = static_cast<int>(h->steps & 1U);
+    const double force[3] = {h->cfg.body_force_x, h->cfg.body_force_y, h->cfg.body_force_z};
+    int rc = with_store(*h, [&]<class T>() -> int {
+        const hipError_t err = lbm::d3q19_moments<T>(descriptor<T>(*h), phase, force, planes, h->stream);
+        return err == hipSuccess ? ok() : hip_fail("moments", err);
+    });
+    double* outs[4] = {rho, ux, uy, uz};
+    for (int k = 0; k < 4 && rc == LBM_OK; ++k) {
+        if (outs[k] != nullptr) {
+            e = hipMemcpyAsync(outs[k], planes + k * h->cells, h->cells * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream);
+            if (e != hipSuccess) {
+                rc = hip_fail("moment readback", e);
+            }
+        }
+    }
+    const int s = sync(*h, "moment readback");
+    (void)hipFree(planes);
+    return rc != LBM_OK ? rc : s;
+}
+
+int lbm3d_populations(lbm3d* h, double* f) {
+    if (h == nullptr || f == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    double* buffer = nullptr;
+    const std::size_t bytes = 19U * h->cells * sizeof(double);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&buffer), bytes);
+    if (e != hipSuccess) {
+        return fail(LBM_ENOMEM, std::string("lbm3d gpu: population readback buffer: ") + hipGetErrorString(e));
+    }
+    const int phase = static_cast<int>(h->steps & 1U);
+    int rc = with_store(*h, [&]<class T>() -> int {
+        const hipError_t err = lbm::d3q19_populations<T>(descriptor<T>(*h), phase, buffer, h->stream);
+        return err == hipSuccess ? ok() : hip_fail("populations", err);
+    });
+    if (rc == LBM_OK) {
+        e = hipMemcpyAsync(f, buffer, bytes, hipMemcpyDeviceToHost, h->stream);
+        if (e != hipSuccess) {
+            rc = hip_fail("population readback", e);
+        }
+    }
+    const int s = sync(*h, "population readback");
+    (void)hipFree(buffer);
+    return rc != LBM_OK ? rc : s;
+}
+
+int lbm3d_links(const lbm3d* h, uint32_t* cell, uint32_t* direction, double* q, size_t capacity, size_t* count) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    const auto& links = h->tables.links;
+    for (std::size_t k = 0; k < links.size() && k < capacity; ++k) {
+        if (cell != nullptr) cell[k] = links[k].cell;
+        if (direction != nullptr) direction[k] = links[k].direction;
+        if (q != nullptr) q[k] = links[k].q;
+    }
+    if (count != nullptr) {
+        *count = links.size();
+    }
+    return ok();
+}
+
+int lbm3d_device_bytes(const lbm3d* h, size_t* bytes) {
+    if (h == nullptr || bytes == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    *bytes = h->device_bytes;
+    return ok();
+}
+
+int lbm3d_synchronize(lbm3d* h) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    return sync(*h, "synchronize");
+}
+
+int lbm3d_set_launch(lbm3d* h, int32_t block, int32_t cells_per_thread) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    if (block != 0 && (block < 64 || block > 256 || block % 64 != 0)) {
+        return fail(LBM_EINVAL, "lbm3d: block size must be 64, 128, 192 or 256");
+    }
+    if (cells_per_thread != 0 && cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4) {
+        return fail(LBM_EINVAL, "lbm3d: cells per thread must be 1, 2 or 4");
+    }
+    if (block != 0) h->block = block;
+    if (cells_per_thread != 0) h->cpt = cells_per_thread;
+    return ok();
+}
+
+int lbm3d_step_bytes(const lbm3d* h, double* bytes) {
+    if (h == nullptr || bytes == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    *bytes = 2.0 * 19.0 * static_cast<double>(store_bytes(h->storage)) * static_cast<double>(h->cells);
+    return ok();
+}
+
+int lbm3d_counts(const lbm3d* h, size_t* bulk, size_t* boundary, size_t* solid, size_t* links) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    std::size_t n[3] = {0, 0, 0};
+    for (const std::uint8_t c : h->tables.cls) {
+        ++n[c];
+    }
+    if (bulk != nullptr) *bulk = n[1];
+    if (boundary != nullptr) *boundary = n[2];
+    if (solid != nullptr) *solid = n[0];
+    if (links != nullptr) *links = h->tables.links.size();
+    return ok();
+}
+
+int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_ms, double* bulk_ms,
+                     double* boundary_ms) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    const std::size_t marks = per_kernel ? 3U * count + 1U : 2U;
+    std::vector<hipEvent_t> ev(marks, nullptr);
+    int rc = LBM_OK;
+    for (auto& e : ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            rc = fail(LBM_EDEVICE, "lbm3d gpu: event creation failed");
+            break;
+        }
+    }
+    if (rc == LBM_OK) {
+        hipError_t e = hipEventRecord(ev[0], h->stream);
+        for (std::uint64_t n = 0; n < count && e == hipSuccess && rc == LBM_OK; ++n) {
+            const bool local = (h->steps & 1U) != 0;
+            rc = with_store(*h, [&]<class T>() -> int {
+                const auto p = descriptor<T>(*h);
+                hipError_t err = lbm::d3q19_step_bulk<T>(p, local, h->block, h->cpt, h->stream);
+                if (err == hipSuccess && per_kernel) {
+                    err = hipEventRecord(ev[3U * n + 1U], h->stream);
+                }
+                if (err == hipSuccess && per_kernel) {
+                    err = hipEventRecord(ev[3U * n + 2U], h->stream);
+                }
+                if (err == hipSuccess) {
+                    err = lbm::d3q19_step_boundary<T>(p, local, h->stream);
+                }
+                if (err == hipSuccess && per_kernel) {
+                    err = hipEventRecord(ev[3U * n + 3U], h->stream);
+                }
+                return err == hipSuccess ? ok() : hip_fail("timed step", err);
+            });
+            ++h->steps;
+        }
+        if (e == hipSuccess && rc == LBM_OK && !per_kernel) {
+            e = hipEventRecord(ev[1], h->stream);
+        }
+        if (e == hipSuccess) {
+            e = hipEventSynchronize(ev[marks - 1]);
+        }
+        if (e != hipSuccess && rc == LBM_OK) {
+            rc = hip_fail("timing", e);
+        }
+        if (rc == LBM_OK) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[0], ev[marks - 1]);
+            if (step_ms != nullptr) *step_ms = ms;
+            double bulk = 0.0;
+            double bnd = 0.0;
+            if (per_kernel) {
+                for (std::uint64_t n = 0; n < count; ++n) {
+                    float a = 0.0f;
+                    float b = 0.0f;
+                    (void)hipEventElapsedTime(&a, ev[3U * n], ev[3U * n + 1U]);
+                    (void)hipEventElapsedTime(&b, ev[3U * n + 2U], ev[3U * n + 3U]);
+                    bulk += a;
+                    bnd += b;
+                }
+            }
+            if (bulk_ms != nullptr) *bulk_ms = bulk;
+            if (boundary_ms != nullptr) *boundary_ms = bnd;
+        }
+    }
+    for (auto& e : ev) {
+        if (e != nullptr) {
+            (void)hipEventDestroy(e);
+        }
+    }
+    return rc;
+}
+
+} // extern "C"

@@ -1,0 +1,159 @@
+// Device-side definitions shared by the D3Q19 and D2Q9 kernels: population storage formats,
+// the in-place (A-A) addressing, launch-shape helpers and the lattice descriptors.
+//
+// In-place A-A scheme (Bailey, Myre, Walsh, Lilja and Pickett 2009). One array holds 19 (9)
+// planes of `stride` populations. Let n be the number of completed steps.
+//  - n even ("post-A" layout): the post-collision population f*_i(x) sits at A[opp(i)][x].
+//  - n odd  ("post-B" layout): f*_i(x) sits at A[i][x + c_i] (in-domain fluid targets only).
+// Step n even (B step): node x reads its pull values f*_i(x - c_i) at A[opp(i)][x - c_i] and
+// writes f*_i(x) to A[i][x + c_i]. Step n odd (A step): node x reads A[i][x] and writes
+// A[opp(i)][x]. In both, the addresses a node reads are exactly the ones it writes, so one
+// array suffices and no two threads touch the same element.
+//
+// Boundary nodes (class 2: domain faces and neighbours of solids) also keep their 19
+// post-collision values in a double-buffered stash, [2][Q][bstride], written every step
+// (buffer (n+1)&1) and read in the next (buffer n&1). Every boundary rule whose value is not
+// the node's own natural pull -- bounce-back, the interpolated wall, moving walls, slip
+// mirrors and the outlet -- reads the stash, so in-place overwrites never race with them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lbm_lattice.h"
+
+namespace lbm {
+
+// ---- storage formats ----------------------------------------------------------------------
+// Each format: S the stored type, R the arithmetic type, `shifted` whether f - w is stored.
+
+struct StoreF32 {
+    using S = float;
+    using R = float;
+    static constexpr bool shifted = true;
+    static constexpr int id = 0;
+    static __device__ __forceinline__ R load(S v) { return v; }
+    static __device__ __forceinline__ S pack(R v) { return v; }
+    static __device__ __forceinline__ S from_double(double v) { return static_cast<float>(v); }
+};
+
+// binary16 holding (f - w) * 2^15: the scale moves the normal range of binary16 down to
+// [1.9e-9, 2.0], which covers every shifted population of a weakly compressible flow.
+struct StoreF16 {
+    using S = _Float16;
+    using R = float;
+    static constexpr bool shifted = true;
+    static constexpr int id = 1;
+    static constexpr float kScale = 32768.0f;
+    static __device__ __forceinline__ R load(S v) { return static_cast<float>(v) * (1.0f / kScale); }
+    static __device__ __forceinline__ S pack(R v) { return static_cast<_Float16>(v * kScale); }
+    static __device__ __forceinline__ S from_double(double v) {
+        return static_cast<_Float16>(static_cast<float>(v * 32768.0));
+    }
+};
+
+// bfloat16 holding f - w, rounded to nearest even.
+struct StoreBF16 {
+    using S = std::uint16_t;
+    using R = float;
+    static constexpr bool shifted = true;
+    static constexpr int id = 2;
+    static __device__ __forceinline__ R load(S v) {
+        return __uint_as_float(static_cast<std::uint32_t>(v) << 16U);
+    }
+    static __device__ __forceinline__ S pack(R v) {
+        const std::uint32_t u = __float_as_uint(v);
+        return static_cast<S>((u + 0x7FFFU + ((u >> 16U) & 1U)) >> 16U);
+    }
+    static __device__ __forceinline__ S from_double(double v) { return pack(static_cast<float>(v)); }
+};
+
+// binary64, unshifted: the on-device oracle, with the reference CPU solver's arithmetic.
+struct StoreF64 {
+    using S = double;
+    using R = double;
+    static constexpr bool shifted = false;
+    static constexpr int id = 3;
+    static __device__ __forceinline__ R load(S v) { return v; }
+    static __device__ __forceinline__ S pack(R v) { return v; }
+    static __device__ __forceinline__ S from_double(double v) { return v; }
+};
+
+// ---- boundary-node source codes -------------------------------------------------------------
+// One 32-bit word per (boundary node, direction), built on the host from resolve_d3q19:
+//   bits 0..2 kind (SourceKind), bit 3 mirror_y applied, bit 4 mirror_z applied,
+//   bits 5..31 payload: kNode unmirrored -> source cell; kNode mirrored / kOutlet -> source
+//   boundary slot; kWall -> side (0..3).
+constexpr std::uint32_t kCodeMirrorY = 8U;
+constexpr std::uint32_t kCodeMirrorZ = 16U;
+constexpr int kCodeShift = 5;
+
+__host__ __device__ constexpr std::uint32_t make_code(int kind, bool my, bool mz,
+                                                      std::uint32_t payload) {
+    return static_cast<std::uint32_t>(kind) | (my ? kCodeMirrorY : 0U) | (mz ? kCodeMirrorZ : 0U) |
+           (payload << kCodeShift);
+}
+
+// ---- lattice descriptor (kernel argument) ---------------------------------------------------
+template <typename R> struct Lattice3 {
+    void* f;                  // S[19][stride]
+    const std::uint8_t* cls;  // node class per cell: 0 solid, 1 bulk, 2 boundary
+    const std::uint32_t* bcell;  // boundary node -> cell [nb]
+    const std::uint32_t* bcode;  // [19][bstride]
+    const R* bq;                 // [19][bstride] wall fractions (interpolated wall) or null
+    R* stash;                    // [2][19][bstride]
+    R* hist;                     // [5][bstride] convective-outlet history (outlet nodes)
+    std::uint32_t nx, ny, nz, cells, stride, nb, bstride;
+    FastDiv div_nx, div_plane;
+    int x_mode;       // 0 periodic, 1 inflow/outflow
+    int convective;   // outlet blends with history
+    int side[4];      // y lo, y hi, z lo, z hi wall codes
+    R wall_u[4];      // wall speeds along x
+    R inlet[3];       // imposed inlet velocity
+    R tau;            // relaxation time
+    R omega;          // 1 / tau (closure off)
+    R smag_c;         // Smagorinsky constant (CPU form: K c c)
+    R smag_c2;        // its square (shifted form: K c^2)
+    int les;
+    R force[3];
+    int forced;
+    R outlet_speed;
+    R sponge_strength;
+    std::uint32_t sponge_in, sponge_out, sponge_sides;
+    int sponge_y, sponge_z;
+};
+
+template <typename R> struct Lattice2 {
+    void* f;                     // S[9][stride]
+    const std::uint8_t* cls;
+    const std::uint32_t* bcell;
+    const std::uint32_t* bcode;  // [9][bstride]
+    R* stash;                    // [2][9][bstride]
+    R* hist;                     // [3][bstride]
+    std::uint32_t nx, ny, cells, stride, nb, bstride;
+    FastDiv div_nx;
+    int x_mode;
+    int convective;
+    int side[2];
+    R wall_u[2];
+    R inlet[2];
+    R tau, omega, smag_c, smag_c2;
+    int les;
+    R force[2];
+    int forced;
+    R outlet_speed;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (block b runs on XCD
+// b % 8). Renumbering them so that each XCD owns one contiguous run of tiles keeps the cache
+// lines shared by neighbouring tiles inside one XCD's L2.
+__device__ __forceinline__ std::uint32_t xcd_tile(std::uint32_t b, std::uint32_t nblocks) {
+    const std::uint32_t q = nblocks >> 3U;
+    const std::uint32_t r = nblocks & 7U;
+    const std::uint32_t xcd = b & 7U;
+    const std::uint32_t local = b >> 3U;
+    return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
+} // namespace lbm

@@ -1,0 +1,224 @@
+"""D3Q19 device solver against the oracle, through the C ABI.
+
+The reference pins its GPU path with GPU-vs-CPU comparisons at relative L2 < 1e-4
+(tests/aero/lbm3d/test_lbm3d_gpu.cpp:197-324); the same cases run here for the binary32
+storage. The binary64 storage runs the CPU solver's arithmetic on the device, so it must agree
+with the oracle to rounding (relative 1e-12) on every boundary rule, which pins the in-place
+A-A addressing and the boundary stash exactly.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cases import (MAGIC_TAU, couette3d_config, duct_config, l2_relative, periodic_box, poiseuille3d_config,
+                   sphere_channel, uniform_state3d, uniform_state_error, velocity_l2, wall)
+from oracle import Oracle3D
+from orrery_amd import (Lbm3DSolver, LbmError, Outlet, Smagorinsky, SolidWall, Sponge, WallKind, box_mask,
+                        fluid_mask_3d)
+
+pytestmark = pytest.mark.gpu
+
+
+def compare(config, mask, steps, storage="f32", label="", init=None):
+    cpu = Oracle3D(config, mask)
+    gpu = Lbm3DSolver(config, mask, storage)
+    if init is not None:
+        cpu.initialize(init)
+        gpu.initialize(init)
+    cpu.step(steps)
+    gpu.step(steps)
+    c = cpu.macroscopic()
+    g = gpu.macroscopic()
+    out = {
+        "velocity": velocity_l2(g, c),
+        "populations": l2_relative(gpu.populations(), cpu.populations()),
+    }
+    fc = cpu.force()
+    fg = gpu.force()
+    out["force_x"] = abs(fg.force[0] - fc.force[0]) / abs(fc.force[0]) if fc.force[0] != 0 else abs(fg.force[0])
+    out["mass"] = abs(gpu.mass() - cpu.mass()) / cpu.mass()
+    print(f"measured: GPU {storage} vs oracle {label} after {steps} steps: {out}", flush=True)
+    return out, cpu, gpu
+
+
+def test_analytic_flows_agree():
+    # test_lbm3d_gpu.cpp:197-209
+    for cfg, mask, label in [
+        (poiseuille3d_config(4, 32, 4, 0.8, 1e-6), fluid_mask_3d(4, 32, 4), "Poiseuille"),
+        (duct_config(4, 20, 20, 0.8, 1e-6), fluid_mask_3d(4, 20, 20), "duct"),
+        (couette3d_config(4, 4, 16, 0.8, 0.02), fluid_mask_3d(4, 4, 16), "Couette"),
+    ]:
+        out, _, _ = compare(cfg, mask, 4000, label=label)
+        assert out["velocity"] < 1e-4
+
+
+SPHERE = sphere_channel(64, 32, 32, 8.0, 2.5, 0.05, 20.0)
+
+
+def sphere_variants():
+    base_c, mask = SPHERE
+    les = base_c.copy()
+    les.smagorinsky = Smagorinsky(True, 0.1)
+    interp = base_c.copy()
+    interp.solid_wall = SolidWall.interpolated
+    extrap = base_c.copy()
+    extrap.outlet = Outlet.extrapolation
+    extrap.y_low = wall(WallKind.periodic)
+    extrap.y_high = wall(WallKind.periodic)
+    return {"bgk": base_c, "les": les, "interpolated": interp, "extrapolation_periodic_y": extrap}, mask
+
+
+@pytest.mark.parametrize("variant", ["bgk", "les", "interpolated", "extrapolation_periodic_y"])
+def test_laminar_sphere_channel_f32(variant):
+    # test_lbm3d_gpu.cpp:211-238: Re 20, D 8, 64x32x32, 800 steps, every metric < 1e-4
+    configs, mask = sphere_variants()
+    out, _, _ = compare(configs[variant], mask, 800, "f32", variant)
+    assert out["velocity"] < 1e-4
+    assert out["populations"] < 1e-4
+    assert out["force_x"] < 1e-4
+
+
+@pytest.mark.parametrize("variant", ["bgk", "les", "interpolated", "extrapolation_periodic_y"])
+def test_laminar_sphere_channel_f64_matches_the_oracle_to_rounding(variant):
+    configs, mask = sphere_variants()
+    out, cpu, gpu = compare(configs[variant], mask, 201, "f64", variant)
+    assert out["velocity"] < 1e-12
+    assert out["populations"] < 1e-13
+    assert out["force_x"] < 1e-11
+    assert out["mass"] < 1e-13
+    fc, fg = cpu.force(), gpu.force()
+    for a in range(3):
+        assert fg.moment[a] == pytest.approx(fc.moment[a], rel=1e-10, abs=1e-12)
+
+
+def test_every_wall_kind_f64_matches_the_oracle_to_rounding():
+    # Moving no-slip walls on z, slip on y, a sphere with the interpolated wall, LES, a
+    # sponge at the inlet and the outlet, and the convective outlet: all rules at once, odd
+    # and even step counts.
+    c, mask = sphere_channel(40, 18, 16, 6.0, 2.0, 0.04, 30.0)
+    c.z_low = wall(WallKind.no_slip, 0.01)
+    c.z_high = wall(WallKind.no_slip, -0.02)
+    c.smagorinsky = Smagorinsky(True, 0.15)
+    c.solid_wall = SolidWall.interpolated
+    c.sponge = Sponge(0.1, 4, 6, 3)
+    for steps in (1, 2, 77):
+        out, cpu, gpu = compare(c, mask, steps, "f64", f"all walls {steps}")
+        assert out["populations"] < 1e-13
+        assert out["velocity"] < 1e-12
+        assert out["force_x"] < 1e-10
+
+
+def test_perturbed_start_is_identical_on_device_and_host():
+    c, mask = sphere_channel(32, 16, 16, 6.0, 2.0, 0.05, 50.0)
+    cpu = Oracle3D(c, mask)
+    gpu = Lbm3DSolver(c, mask, "f64")
+    cpu.initialize_perturbed(1234, 1e-4)
+    gpu.initialize_perturbed(1234, 1e-4)
+    assert np.array_equal(cpu.populations(), gpu.populations())
+    cpu.step(10)
+    gpu.step(10)
+    assert l2_relative(gpu.populations(), cpu.populations()) < 1e-14
+
+
+def test_unusable_wall_distances_agree():
+    # test_lbm3d_gpu.cpp:240-268
+    c, mask = sphere_channel(48, 24, 24, 8.0, 2.0, 0.05, 20.0)
+    c.solid_wall = SolidWall.interpolated
+    centre = 2.0 * 8.0
+    x = np.tile(np.arange(48, dtype=np.float64), 24 * 24)
+    front = x < centre
+    d = mask.distance.copy()
+    d[front & (mask.solid == 0)] = math.inf
+    d[~front & (mask.solid != 0)] = 1.0
+    mask.distance = d
+    out, _, _ = compare(c, mask, 400, "f32", "unusable distances")
+    assert out["velocity"] < 1e-4
+    assert out["populations"] < 1e-4
+    assert out["force_x"] < 1e-4
+
+
+@pytest.mark.parametrize("rho", [0.7, 2.0])
+def test_co_moving_walls_f32(rho):
+    # test_lbm3d_gpu.cpp:270-289
+    c = periodic_box(4, 8, 8, 0.7)
+    u = 0.03
+    for side in ("y_low", "y_high", "z_low", "z_high"):
+        setattr(c, side, wall(WallKind.no_slip, u))
+    g = Lbm3DSolver(c, fluid_mask_3d(4, 8, 8))
+    g.initialize(uniform_state3d(c, rho, u))
+    g.step(300)
+    assert uniform_state_error(g.macroscopic(), rho, (u, 0.0, 0.0), u) < 1e-4
+
+
+def test_call_size_independence_is_bitwise():
+    c, mask = sphere_channel(32, 16, 16, 6.0, 2.0, 0.05, 50.0)
+    a = Lbm3DSolver(c, mask)
+    b = Lbm3DSolver(c, mask)
+    a.step(150)
+    b.step(100)
+    b.step(50)
+    assert np.array_equal(a.populations(), b.populations())
+    assert a.force() == b.force()
+
+
+def test_ramped_inlet_agrees():
+    c, mask = sphere_channel(32, 16, 16, 6.0, 2.0, 0.05, 50.0)
+    cpu = Oracle3D(c, mask)
+    gpu = Lbm3DSolver(c, mask, "f64")
+    for n in range(5):
+        v = (0.05 * (1 - math.cos(math.pi * n / 5)) / 2, 0.0, 0.0)
+        cpu.set_inlet_velocity(v)
+        gpu.set_inlet_velocity(v)
+        cpu.step(7)
+        gpu.step(7)
+    assert l2_relative(gpu.populations(), cpu.populations()) < 1e-13
+
+
+def test_momentum_balance_on_device():
+    # test_lbm3d_flows.cpp:170-188 on the device (binary32): force = body force x fluid cells
+    c = periodic_box(4, 16, 4, 0.9)
+    c.body_force_x = 2e-6
+    g = Lbm3DSolver(c, box_mask(4, 16, 4, (0.0, 0.0, 0.0), (3.0, 0.0, 3.0)))
+    g.step(6000)
+    f = g.force()
+    assert f.force[0] == pytest.approx(2e-6 * 240, rel=2e-3)
+
+
+def test_mass_conservation_f64():
+    from test_oracle_flows import mass_case
+    c, mask, planes = mass_case()
+    g = Lbm3DSolver(c, mask, "f64")
+    g.initialize(planes)
+    before = g.mass()
+    g.step(1500)
+    assert abs(g.mass() - before) / before < 1e-12
+
+
+def test_refusals_happen_before_allocation():
+    c = periodic_box(8, 8, 8, 0.5 + 1e-9)
+    with pytest.raises(LbmError, match="rounds to 1/2 in binary32"):
+        Lbm3DSolver(c, fluid_mask_3d(8, 8, 8))
+    Lbm3DSolver(c, fluid_mask_3d(8, 8, 8), "f64")  # binary64 keeps the viscosity
+    from orrery_amd import set_memory_limit
+    set_memory_limit(1024)
+    try:
+        with pytest.raises(LbmError, match="are available; use fewer cells"):
+            Lbm3DSolver(periodic_box(16, 16, 16, 0.6), fluid_mask_3d(16, 16, 16))
+    finally:
+        set_memory_limit(0)
+
+
+@pytest.mark.parametrize("storage,limit", [("f16s", 5e-3), ("bf16s", 5e-2)])
+def test_sixteen_bit_storage_error_against_fp64(storage, limit):
+    configs, mask = sphere_variants()
+    c = configs["bgk"]
+    ref = Lbm3DSolver(c, mask, "f64")
+    low = Lbm3DSolver(c, mask, storage)
+    ref.step(800)
+    low.step(800)
+    v = velocity_l2(low.macroscopic(), ref.macroscopic())
+    f = abs(low.force().force[0] - ref.force().force[0]) / abs(ref.force().force[0])
+    print(f"measured: {storage} vs f64 sphere Re 20 after 800 steps: velocity L2 {v:.3e}, force_x {f:.3e}")
+    assert v < limit
+    assert f < limit
