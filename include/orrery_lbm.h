@@ -168,6 +168,20 @@ int lbm2d_links(const lbm2d* h, uint32_t* cell, uint32_t* direction, size_t capa
                 size_t* count);
 int lbm2d_synchronize(lbm2d* h);
 
+/* ---- host-only setup (no device needed) --------------------------------------------------- */
+/* validate() (src/aero/lbm3d/lbm3d_common.cpp:159-161): LBM_OK, or LBM_EINVAL with the
+ * reference's reason in lbm_last_error(). */
+int lbm3d_validate(const lbm3d_config* config, const uint8_t* solid, const double* distance);
+int lbm2d_validate(const lbm2d_config* config, const uint8_t* solid);
+/* Node classes (0 solid, 1 bulk, 2 boundary; classify_nodes, lbm3d_internal.h:219-248). */
+int lbm3d_classify(const lbm3d_config* config, const uint8_t* solid, uint8_t* classes);
+/* boundary_links (lbm3d_common.cpp:176-198, lbm2d_common.cpp:79-97) of a validated
+ * configuration, in the reference's order, as lbm3d_links/lbm2d_links report them. */
+int lbm3d_boundary_links(const lbm3d_config* config, const uint8_t* solid, const double* distance,
+                         uint32_t* cell, uint32_t* direction, double* q, size_t capacity, size_t* count);
+int lbm2d_boundary_links(const lbm2d_config* config, const uint8_t* solid, uint32_t* cell,
+                         uint32_t* direction, size_t capacity, size_t* count);
+
 /* ---- shared helpers ----------------------------------------------------------------------- */
 /* Seeded perturbation in [-1, 1): SplitMix64 of (seed, cell, component), 53-bit mantissa.
  * The same counter-based draw on the host and the device, so every solver starts alike. */

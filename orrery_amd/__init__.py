@@ -8,7 +8,7 @@ from .config import (Lbm2DConfig, Lbm3DConfig, Outlet, Smagorinsky, SolidWall, S
                      XBoundary, tau_of_viscosity, viscosity_of_tau)
 from .masks import (SolidMask2D, SolidMask3D, box_mask, circle_mask, fluid_mask, fluid_mask_3d,
                     mask_from_sdf, mask_from_sdf_3d, sphere_mask)
-from .solver import (BoundaryLinks, Force2D, Force3D, Lbm2DSolver, Lbm3DSolver, LbmError, MacroFields2D,
+from .solver import (BoundaryLinks, Force2D, Force3D, Lbm2DSolver, Lbm3DSolver, LbmError, MacroFields2D, boundary_links, classify_nodes, validate,
                      MacroFields3D, device_memory, make_hip_solver, set_memory_limit)
 
 __all__ = [
@@ -16,5 +16,6 @@ __all__ = [
     "XBoundary", "tau_of_viscosity", "viscosity_of_tau", "SolidMask2D", "SolidMask3D", "box_mask",
     "circle_mask", "fluid_mask", "fluid_mask_3d", "mask_from_sdf", "mask_from_sdf_3d", "sphere_mask",
     "BoundaryLinks", "Force2D", "Force3D", "Lbm2DSolver", "Lbm3DSolver", "LbmError", "MacroFields2D",
-    "MacroFields3D", "device_memory", "make_hip_solver", "set_memory_limit",
+    "MacroFields3D", "device_memory", "make_hip_solver", "set_memory_limit", "boundary_links", "classify_nodes",
+    "validate",
 ]

@@ -93,6 +93,12 @@ EXPORTS = [
     ("lbm2d_populations", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm2d_links", ctypes.c_int, [ctypes.c_void_p, c_u32p, c_u32p, ctypes.c_size_t, c_sizep]),
     ("lbm2d_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm3d_validate", ctypes.c_int, [P(CConfig3D), c_u8p, c_f64p]),
+    ("lbm2d_validate", ctypes.c_int, [P(CConfig2D), c_u8p]),
+    ("lbm3d_classify", ctypes.c_int, [P(CConfig3D), c_u8p, c_u8p]),
+    ("lbm3d_boundary_links", ctypes.c_int, [P(CConfig3D), c_u8p, c_f64p, c_u32p, c_u32p, c_f64p, ctypes.c_size_t,
+                                            c_sizep]),
+    ("lbm2d_boundary_links", ctypes.c_int, [P(CConfig2D), c_u8p, c_u32p, c_u32p, ctypes.c_size_t, c_sizep]),
     ("lbm_perturbation", ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]),
 ]
 

@@ -528,6 +528,70 @@ int lbm_set_memory_limit(size_t bytes) {
     return LBM_OK;
 }
 
+int lbm3d_validate(const lbm3d_config* config, const uint8_t* solid, const double* distance) {
+    if (config == nullptr) {
+        return lbm::host::fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    const std::string p = lbm::host::problem3(*config, solid, distance);
+    return p.empty() ? LBM_OK : lbm::host::fail(LBM_EINVAL, p);
+}
+
+int lbm2d_validate(const lbm2d_config* config, const uint8_t* solid) {
+    if (config == nullptr) {
+        return lbm::host::fail(LBM_EINVAL, "lbm2d: null argument");
+    }
+    const std::string p = lbm::host::problem2(*config, solid);
+    return p.empty() ? LBM_OK : lbm::host::fail(LBM_EINVAL, p);
+}
+
+int lbm3d_classify(const lbm3d_config* config, const uint8_t* solid, uint8_t* classes) {
+    if (config == nullptr || classes == nullptr) {
+        return lbm::host::fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    // Node classes do not depend on the wall model: validate and classify with the halfway wall.
+    lbm3d_config c = *config;
+    c.solid_wall = LBM_SOLID_HALFWAY;
+    if (int rc = lbm3d_validate(&c, solid, nullptr); rc != LBM_OK) {
+        return rc;
+    }
+    const lbm::host::Tables t = lbm::host::tables3(c, solid, nullptr);
+    std::copy(t.cls.begin(), t.cls.end(), classes);
+    return LBM_OK;
+}
+
+int lbm3d_boundary_links(const lbm3d_config* config, const uint8_t* solid, const double* distance, uint32_t* cell,
+                         uint32_t* direction, double* q, size_t capacity, size_t* count) {
+    if (int rc = lbm3d_validate(config, solid, distance); rc != LBM_OK) {
+        return rc;
+    }
+    const lbm::host::Tables t = lbm::host::tables3(*config, solid, distance);
+    for (std::size_t k = 0; k < t.links.size() && k < capacity; ++k) {
+        if (cell != nullptr) cell[k] = t.links[k].cell;
+        if (direction != nullptr) direction[k] = t.links[k].direction;
+        if (q != nullptr) q[k] = t.links[k].q;
+    }
+    if (count != nullptr) {
+        *count = t.links.size();
+    }
+    return LBM_OK;
+}
+
+int lbm2d_boundary_links(const lbm2d_config* config, const uint8_t* solid, uint32_t* cell, uint32_t* direction,
+                         size_t capacity, size_t* count) {
+    if (int rc = lbm2d_validate(config, solid); rc != LBM_OK) {
+        return rc;
+    }
+    const lbm::host::Tables t = lbm::host::tables2(*config, solid);
+    for (std::size_t k = 0; k < t.links.size() && k < capacity; ++k) {
+        if (cell != nullptr) cell[k] = t.links[k].cell;
+        if (direction != nullptr) direction[k] = t.links[k].direction;
+    }
+    if (count != nullptr) {
+        *count = t.links.size();
+    }
+    return LBM_OK;
+}
+
 // SplitMix64 finaliser (Steele, Lea and Flood 2014) keyed by (seed, cell, component); the
 // device copy sits in the init kernels.
 double lbm_perturbation(uint64_t seed, uint64_t cell, uint32_t component) {

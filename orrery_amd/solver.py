@@ -324,3 +324,52 @@ def make_hip_solver(config, mask, storage: str = "f32"):
     if isinstance(config, Lbm3DConfig):
         return Lbm3DSolver(config, mask, storage)
     return Lbm2DSolver(config, mask, storage)
+
+
+# ---- host-only setup (no device) -----------------------------------------------------------
+def validate(config, mask) -> str | None:
+    """The reference's validate(): None when the configuration can run, else the reason."""
+    lib = abi.load()
+    c = config.to_c()
+    solid = np.ascontiguousarray(mask.solid, dtype=np.uint8)
+    if isinstance(config, Lbm3DConfig):
+        distance = None if mask.distance is None else np.ascontiguousarray(mask.distance, dtype=np.float64)
+        rc = lib.lbm3d_validate(ctypes.byref(c), abi.ptr(solid, ctypes.c_uint8), abi.ptr(distance, ctypes.c_double))
+    else:
+        rc = lib.lbm2d_validate(ctypes.byref(c), abi.ptr(solid, ctypes.c_uint8))
+    return None if rc == abi.LBM_OK else lib.lbm_last_error().decode()
+
+
+def classify_nodes(config: Lbm3DConfig, mask: SolidMask3D) -> np.ndarray:
+    """Node classes 0 solid, 1 bulk, 2 boundary (classify_nodes, lbm3d_internal.h:219-248)."""
+    lib = abi.load()
+    c = config.to_c()
+    out = np.empty(config.cells, dtype=np.uint8)
+    solid = np.ascontiguousarray(mask.solid, dtype=np.uint8)
+    _check(lib, lib.lbm3d_classify(ctypes.byref(c), abi.ptr(solid, ctypes.c_uint8), abi.ptr(out, ctypes.c_uint8)))
+    return out
+
+
+def boundary_links(config, mask) -> BoundaryLinks:
+    """boundary_links() of the reference, computed by the library's host setup."""
+    lib = abi.load()
+    c = config.to_c()
+    solid = np.ascontiguousarray(mask.solid, dtype=np.uint8)
+    n = ctypes.c_size_t()
+    if isinstance(config, Lbm3DConfig):
+        distance = None if mask.distance is None else np.ascontiguousarray(mask.distance, dtype=np.float64)
+        args = (ctypes.byref(c), abi.ptr(solid, ctypes.c_uint8), abi.ptr(distance, ctypes.c_double))
+        _check(lib, lib.lbm3d_boundary_links(*args, None, None, None, 0, ctypes.byref(n)))
+        cell = np.empty(n.value, dtype=np.uint32)
+        d = np.empty(n.value, dtype=np.uint32)
+        q = np.empty(n.value)
+        _check(lib, lib.lbm3d_boundary_links(*args, abi.ptr(cell, ctypes.c_uint32), abi.ptr(d, ctypes.c_uint32),
+                                             abi.ptr(q, ctypes.c_double), n.value, ctypes.byref(n)))
+        return BoundaryLinks(cell, d, q)
+    args = (ctypes.byref(c), abi.ptr(solid, ctypes.c_uint8))
+    _check(lib, lib.lbm2d_boundary_links(*args, None, None, 0, ctypes.byref(n)))
+    cell = np.empty(n.value, dtype=np.uint32)
+    d = np.empty(n.value, dtype=np.uint32)
+    _check(lib, lib.lbm2d_boundary_links(*args, abi.ptr(cell, ctypes.c_uint32), abi.ptr(d, ctypes.c_uint32),
+                                         n.value, ctypes.byref(n)))
+    return BoundaryLinks(cell, d, np.full(n.value, 0.5))
