@@ -182,6 +182,31 @@ int lbm3d_boundary_links(const lbm3d_config* config, const uint8_t* solid, const
 int lbm2d_boundary_links(const lbm2d_config* config, const uint8_t* solid, uint32_t* cell,
                          uint32_t* direction, size_t capacity, size_t* count);
 
+/* ---- force-coefficient history (src/aero/lbm/include/orrery/aero/lbm/force_history.h) ---- */
+typedef struct lbm_oscillation { /* Oscillation, force_history.h:48-54 */
+    double period;
+    double frequency;
+    size_t cycles;
+    double amplitude;
+    double period_spread;
+} lbm_oscillation;
+/* force_coefficients (force_history.cpp:9-13): C = F / (U^2 L / 2), lattice units. */
+void lbm_force_coefficients(double force_x, double force_y, double speed, double length, double* drag,
+                            double* lift);
+/* mean_since (force_history.cpp:36-47). */
+double lbm_mean_since(const double* time, const double* values, size_t n, double from);
+/* dominant_oscillation (force_history.cpp:49-100): 1 and *out set when at least two complete
+ * periods of hysteresis mean crossings are found, else 0. */
+int lbm_dominant_oscillation(const double* time, const double* values, size_t n, double from,
+                             lbm_oscillation* out);
+/* steady_since (force_history.cpp:149-157) with SettledCriteria::steady_tolerance. */
+int lbm_steady_since(const double* time, const double* drag, const double* lift, size_t n, double from,
+                     double steady_tolerance);
+/* settled_oscillation (force_history.cpp:159-193). */
+int lbm_settled_oscillation(const double* time, const double* drag, const double* lift, size_t n,
+                            double from, double max_period_spread, double stationarity,
+                            lbm_oscillation* out);
+
 /* ---- shared helpers ----------------------------------------------------------------------- */
 /* Seeded perturbation in [-1, 1): SplitMix64 of (seed, cell, component), 53-bit mantissa.
  * The same counter-based draw on the host and the device, so every solver starts alike. */

@@ -244,3 +244,48 @@ double ref_lattice_tau(double length, double velocity, double viscosity, double 
 }
 
 } // extern "C"
+
+#include "orrery/aero/lbm/force_history.h"
+
+extern "C" {
+
+int ref_dominant_oscillation(const double* time, const double* values, std::size_t n, double from, double* out5) {
+    const auto o = dominant_oscillation(std::span<const double>(time, n), std::span<const double>(values, n), from);
+    if (!o) {
+        return 0;
+    }
+    out5[0] = o->period;
+    out5[1] = o->frequency;
+    out5[2] = static_cast<double>(o->cycles);
+    out5[3] = o->amplitude;
+    out5[4] = o->period_spread;
+    return 1;
+}
+
+int ref_settled_oscillation(const double* time, const double* drag, const double* lift, std::size_t n, double from,
+                            double* out5) {
+    const auto o = settled_oscillation(std::span<const double>(time, n), std::span<const double>(drag, n),
+                                       std::span<const double>(lift, n), from);
+    if (!o) {
+        return 0;
+    }
+    out5[0] = o->period;
+    out5[1] = o->frequency;
+    out5[2] = static_cast<double>(o->cycles);
+    out5[3] = o->amplitude;
+    out5[4] = o->period_spread;
+    return 1;
+}
+
+int ref_steady_since(const double* time, const double* drag, const double* lift, std::size_t n, double from) {
+    return steady_since(std::span<const double>(time, n), std::span<const double>(drag, n),
+                        std::span<const double>(lift, n), from)
+               ? 1
+               : 0;
+}
+
+double ref_mean_since(const double* time, const double* values, std::size_t n, double from) {
+    return mean_since(std::span<const double>(time, n), std::span<const double>(values, n), from);
+}
+
+} // extern "C"

@@ -50,6 +50,11 @@ class CConfig2D(ctypes.Structure):
     ]
 
 
+class COscillation(ctypes.Structure):
+    _fields_ = [("period", ctypes.c_double), ("frequency", ctypes.c_double), ("cycles", ctypes.c_size_t),
+                ("amplitude", ctypes.c_double), ("period_spread", ctypes.c_double)]
+
+
 P = ctypes.POINTER
 c_u8p = P(ctypes.c_uint8)
 c_u32p = P(ctypes.c_uint32)
@@ -100,6 +105,15 @@ EXPORTS = [
                                             c_sizep]),
     ("lbm2d_boundary_links", ctypes.c_int, [P(CConfig2D), c_u8p, c_u32p, c_u32p, ctypes.c_size_t, c_sizep]),
     ("lbm_perturbation", ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]),
+    ("lbm_force_coefficients", None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, c_f64p,
+                                      c_f64p]),
+    ("lbm_mean_since", ctypes.c_double, [c_f64p, c_f64p, ctypes.c_size_t, ctypes.c_double]),
+    ("lbm_dominant_oscillation", ctypes.c_int, [c_f64p, c_f64p, ctypes.c_size_t, ctypes.c_double,
+                                                P(COscillation)]),
+    ("lbm_steady_since", ctypes.c_int, [c_f64p, c_f64p, c_f64p, ctypes.c_size_t, ctypes.c_double,
+                                        ctypes.c_double]),
+    ("lbm_settled_oscillation", ctypes.c_int, [c_f64p, c_f64p, c_f64p, ctypes.c_size_t, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double, P(COscillation)]),
 ]
 
 _lib = None

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["lbm_setup.hip", "lbm3d_kernels.hip", "lbm3d_abi.hip", "lbm2d_kernels.hip", "lbm2d_abi.hip",
-           "lbm_batch.hip"]
+           "lbm_batch.hip", "lbm_history.cpp"]
 OUTPUT = os.path.join(HERE, "liborrery_lbm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1,0 +1,115 @@
+"""Force-coefficient history: Cd/Cl samples, window means, the shedding frequency (Strouhal
+number) and the convergence tests. Python face of the library's C++ implementation
+(lbm_history.cpp), mirroring src/aero/lbm/include/orrery/aero/lbm/force_history.h:11-96."""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import abi
+
+
+@dataclasses.dataclass
+class Coefficients:
+    drag: float = 0.0
+    lift: float = 0.0
+
+
+@dataclasses.dataclass
+class Oscillation:
+    period: float
+    frequency: float
+    cycles: int
+    amplitude: float
+    period_spread: float
+
+
+@dataclasses.dataclass
+class SettledCriteria:
+    """force_history.h:65-75"""
+    steady_tolerance: float = 1e-3
+    max_period_spread: float = 0.05
+    stationarity: float = 0.02
+
+
+def force_coefficients(force_x: float, force_y: float, speed: float, length: float) -> Coefficients:
+    d = ctypes.c_double()
+    l = ctypes.c_double()
+    abi.load().lbm_force_coefficients(force_x, force_y, speed, length, ctypes.byref(d), ctypes.byref(l))
+    return Coefficients(d.value, l.value)
+
+
+def _arr(v):
+    return np.ascontiguousarray(v, dtype=np.float64)
+
+
+def mean_since(time, values, start: float) -> float:
+    t, v = _arr(time), _arr(values)
+    n = min(len(t), len(v))
+    return abi.load().lbm_mean_since(abi.ptr(t, ctypes.c_double), abi.ptr(v, ctypes.c_double), n, start)
+
+
+def _osc(o: abi.COscillation) -> Oscillation:
+    return Oscillation(o.period, o.frequency, int(o.cycles), o.amplitude, o.period_spread)
+
+
+def dominant_oscillation(time, values, start: float) -> Oscillation | None:
+    t, v = _arr(time), _arr(values)
+    n = min(len(t), len(v))
+    o = abi.COscillation()
+    ok = abi.load().lbm_dominant_oscillation(abi.ptr(t, ctypes.c_double), abi.ptr(v, ctypes.c_double), n, start,
+                                             ctypes.byref(o))
+    return _osc(o) if ok else None
+
+
+def steady_since(time, drag, lift, start: float, criteria: SettledCriteria = SettledCriteria()) -> bool:
+    t, d, l = _arr(time), _arr(drag), _arr(lift)
+    n = min(len(t), len(d), len(l))
+    return bool(abi.load().lbm_steady_since(abi.ptr(t, ctypes.c_double), abi.ptr(d, ctypes.c_double),
+                                            abi.ptr(l, ctypes.c_double), n, start, criteria.steady_tolerance))
+
+
+def settled_oscillation(time, drag, lift, start: float,
+                        criteria: SettledCriteria = SettledCriteria()) -> Oscillation | None:
+    t, d, l = _arr(time), _arr(drag), _arr(lift)
+    n = min(len(t), len(d), len(l))
+    o = abi.COscillation()
+    ok = abi.load().lbm_settled_oscillation(abi.ptr(t, ctypes.c_double), abi.ptr(d, ctypes.c_double),
+                                            abi.ptr(l, ctypes.c_double), n, start, criteria.max_period_spread,
+                                            criteria.stationarity, ctypes.byref(o))
+    return _osc(o) if ok else None
+
+
+class CoefficientHistory:
+    """Samples of (time, Cd, Cl) with a moving average over the last `window` samples
+    (force_history.h:22-42)."""
+
+    def __init__(self, window: int):
+        self._window = max(1, int(window))
+        self.time: list[float] = []
+        self.drag: list[float] = []
+        self.lift: list[float] = []
+
+    def push(self, time: float, sample: Coefficients) -> None:
+        self.time.append(float(time))
+        self.drag.append(float(sample.drag))
+        self.lift.append(float(sample.lift))
+
+    def size(self) -> int:
+        return len(self.time)
+
+    def window(self) -> int:
+        return self._window
+
+    def moving_average(self) -> Coefficients:
+        n = min(self._window, len(self.time))
+        if n == 0:
+            return Coefficients()
+        d = 0.0
+        l = 0.0
+        for k in range(len(self.time) - n, len(self.time)):
+            d += self.drag[k]
+            l += self.lift[k]
+        return Coefficients(d / n, l / n)

@@ -1,0 +1,71 @@
+"""D2Q9 device solver against the oracle through the C ABI (the reference's 2D GPU-vs-CPU tests,
+tests/aero/lbm/test_lbm_gpu.cpp), and the plain 2D channel workload (400 x 160, cylinder D 32)."""
+import numpy as np
+import pytest
+
+from cases import cylinder_channel_2d, l2_relative, velocity_l2
+from oracle import Oracle2D
+from orrery_amd import Lbm2DSolver, Outlet, Smagorinsky, Wall, WallKind
+from test_oracle_2d import periodic_box, poiseuille_config, taylor_green
+
+pytestmark = pytest.mark.gpu
+
+
+def compare(config, mask, steps, storage, init=None):
+    cpu = Oracle2D(config, mask)
+    gpu = Lbm2DSolver(config, mask, storage)
+    if init is not None:
+        cpu.initialize(init)
+        gpu.initialize(init)
+    cpu.step(steps)
+    gpu.step(steps)
+    v = velocity_l2(gpu.macroscopic(), cpu.macroscopic())
+    p = l2_relative(gpu.populations(), cpu.populations())
+    fc, fg = cpu.force(), gpu.force()
+    fx = abs(fg.x - fc.x) / abs(fc.x) if fc.x else abs(fg.x)
+    print(f"measured: D2Q9 {storage} vs oracle after {steps}: velocity {v:.3e} populations {p:.3e} force_x {fx:.3e}")
+    return v, p, fx, cpu, gpu
+
+
+def test_poiseuille_and_couette_agree():
+    v, _, _, _, _ = compare(poiseuille_config(4, 32, 0.8, 1e-6), __import__("orrery_amd").fluid_mask(4, 32), 4000, "f32")
+    assert v < 1e-4
+    c = periodic_box(4, 16, 0.9)
+    c.bottom = Wall(WallKind.no_slip)
+    c.top = Wall(WallKind.no_slip, 0.02)
+    v, _, _, _, _ = compare(c, __import__("orrery_amd").fluid_mask(4, 16), 4000, "f32")
+    assert v < 1e-4
+
+
+@pytest.mark.parametrize("outlet", [Outlet.extrapolation, Outlet.convective])
+def test_cylinder_channel_f32_before_shedding(outlet):
+    c, m = cylinder_channel_2d(outlet=outlet)
+    v, p, fx, _, _ = compare(c, m, 2000, "f32")
+    assert v < 1e-4 and p < 1e-4 and fx < 1e-4
+
+
+@pytest.mark.parametrize("outlet", [Outlet.extrapolation, Outlet.convective])
+def test_cylinder_channel_f64_matches_the_oracle_to_rounding(outlet):
+    c, m = cylinder_channel_2d(outlet=outlet)
+    c.smagorinsky = Smagorinsky(True, 0.1)
+    v, p, fx, cpu, gpu = compare(c, m, 501, "f64")
+    assert v < 1e-12 and p < 1e-13 and fx < 1e-11
+    assert abs(gpu.mass() - cpu.mass()) / cpu.mass() < 1e-13
+
+
+def test_moving_walls_and_taylor_green_f64():
+    c = periodic_box(48, 40, 0.56)
+    c.bottom = Wall(WallKind.no_slip, 0.0)
+    c.top = Wall(WallKind.no_slip, 0.03)
+    from orrery_amd import circle_mask
+    v, p, _, _, _ = compare(c, circle_mask(48, 40, 20.0, 19.0, 6.0), 301, "f64", taylor_green(40, 0.04, c))
+    assert p < 1e-13
+
+
+def test_perturbed_start_matches_the_oracle():
+    c, m = cylinder_channel_2d()
+    cpu = Oracle2D(c, m)
+    gpu = Lbm2DSolver(c, m, "f64")
+    cpu.initialize_perturbed(99, 1e-4)
+    gpu.initialize_perturbed(99, 1e-4)
+    assert np.array_equal(cpu.populations(), gpu.populations())

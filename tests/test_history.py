@@ -1,0 +1,117 @@
+"""Force-coefficient history: the reference's own tests (tests/aero/lbm/test_force_history.cpp)
+restated, and bitwise agreement with the reference's force_history.cpp (oracle/_ref)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import refcheck
+from orrery_amd.history import (CoefficientHistory, Coefficients, dominant_oscillation, force_coefficients,
+                                mean_since, settled_oscillation, steady_since)
+
+
+def test_force_coefficients_use_the_dynamic_pressure():
+    c = force_coefficients(0.0135, -0.002, 0.05, 20.0)
+    assert c.drag == pytest.approx(0.0135 / (0.5 * 0.0025 * 20.0))
+    assert c.lift == pytest.approx(-0.002 / (0.5 * 0.0025 * 20.0))
+
+
+def test_moving_average_covers_the_last_window():
+    h = CoefficientHistory(3)
+    assert h.moving_average().drag == 0.0
+    h.push(0.0, Coefficients(1.0, 0.0))
+    assert h.moving_average().drag == 1.0
+    h.push(1.0, Coefficients(2.0, 1.0))
+    h.push(2.0, Coefficients(3.0, -1.0))
+    h.push(3.0, Coefficients(7.0, 3.0))
+    assert h.size() == 4
+    assert h.moving_average().drag == pytest.approx(4.0)
+    assert h.moving_average().lift == pytest.approx(1.0)
+    assert CoefficientHistory(0).window() == 1
+
+
+def shedding_like():
+    f = 1.0 / 2425.0
+    t = np.arange(0, 40000, 10, dtype=np.float64)
+    v = 1.3 + 0.3 * np.sin(2.0 * math.pi * f * t + 0.4) + np.exp(-t / 500.0)
+    return f, t, v
+
+
+def test_dominant_oscillation_of_a_shedding_like_signal():
+    f, t, v = shedding_like()
+    o = dominant_oscillation(t, v, 10000.0)
+    assert o is not None
+    assert o.frequency == pytest.approx(f, rel=2e-4)
+    assert o.cycles >= 10
+    assert o.amplitude == pytest.approx(0.3, rel=1e-3)
+    assert o.period_spread < 0.01
+    assert mean_since(t, v, 10000.0) == pytest.approx(1.3, rel=1e-2)
+    assert dominant_oscillation(t, np.ones_like(t), 0.0) is None
+
+
+def signal(amplitude, drag, phase):
+    t = np.arange(0, 20000, 10, dtype=np.float64)
+    return t, np.array([drag(x) for x in t]), np.array([amplitude(x) * math.sin(2 * math.pi * phase(x)) for x in t])
+
+
+def settled_cases():
+    constant = lambda x: 0.4  # noqa: E731
+    mean_drag = lambda x: 1.4 + 0.01 * math.sin(4.0 * math.pi * x / 1000.0)  # noqa: E731
+    regular = lambda x: x / 1000.0  # noqa: E731
+    return {
+        "locked": signal(constant, mean_drag, regular),
+        "growing": signal(lambda x: 0.4 * math.exp((x - 20000.0) / 10500.0), mean_drag, regular),
+        "uneven": signal(constant, mean_drag, lambda x: x / 1000.0 + 0.15 * math.sin(2 * math.pi * x / 2700.0)),
+        "drifting": signal(constant, lambda x: 1.5 - 1.5e-5 * x, regular),
+    }
+
+
+def test_settled_shedding_needs_consistent_periods_and_stationarity():
+    cases = settled_cases()
+    t, d, l = cases["locked"]
+    o = settled_oscillation(t, d, l, 10000.0)
+    assert o is not None and o.period == pytest.approx(1000.0, rel=1e-3)
+    for name in ("growing", "uneven", "drifting"):
+        t, d, l = cases[name]
+        assert dominant_oscillation(t, l, 10000.0) is not None
+        assert settled_oscillation(t, d, l, 10000.0) is None, name
+    t, d, l = cases["locked"]
+    assert settled_oscillation(t, d, l, 18500.0) is None
+
+
+def test_steady_history():
+    t = np.arange(2000, dtype=np.float64)
+    drag = 1.2 + 0.5 * np.exp(-t / 100.0)
+    lift = 1e-12 * np.sin(t)
+    assert steady_since(t, drag, lift, 1500.0)
+    assert not steady_since(t, drag, lift, 500.0)
+    assert not steady_since(t, drag, lift, 1999.5)
+    assert not steady_since(t, drag, 0.01 * np.sin(t / 50.0), 1500.0)
+
+
+@pytest.mark.skipif(not refcheck.available(), reason="oracle/_ref not built (reference tree absent)")
+def test_history_matches_the_reference_bitwise():
+    ref = refcheck.load()
+    P = ctypes.POINTER(ctypes.c_double)
+    for name in ("ref_dominant_oscillation", "ref_settled_oscillation", "ref_steady_since", "ref_mean_since"):
+        getattr(ref, name).restype = ctypes.c_double if name == "ref_mean_since" else ctypes.c_int
+    out = np.zeros(5)
+    p = refcheck.ptr
+    _, t, v = shedding_like()
+    for start in (0.0, 5000.0, 10000.0, 39000.0):
+        ok = ref.ref_dominant_oscillation(p(t), p(v), ctypes.c_size_t(len(t)), ctypes.c_double(start), p(out))
+        o = dominant_oscillation(t, v, start)
+        assert bool(ok) == (o is not None)
+        if ok:
+            assert [o.period, o.frequency, float(o.cycles), o.amplitude, o.period_spread] == out.tolist()
+        assert ref.ref_mean_since(p(t), p(v), ctypes.c_size_t(len(t)), ctypes.c_double(start)) == mean_since(t, v, start)
+    for name, (t, d, l) in settled_cases().items():
+        for start in (0.0, 10000.0, 18500.0):
+            ok = ref.ref_settled_oscillation(p(t), p(d), p(l), ctypes.c_size_t(len(t)), ctypes.c_double(start), p(out))
+            o = settled_oscillation(t, d, l, start)
+            assert bool(ok) == (o is not None), (name, start)
+            if ok:
+                assert [o.period, o.frequency, float(o.cycles), o.amplitude, o.period_spread] == out.tolist()
+            assert bool(ref.ref_steady_since(p(t), p(d), p(l), ctypes.c_size_t(len(t)), ctypes.c_double(start))) == \
+                steady_since(t, d, l, start)
