@@ -168,6 +168,41 @@ int lbm2d_links(const lbm2d* h, uint32_t* cell, uint32_t* direction, size_t capa
                 size_t* count);
 int lbm2d_synchronize(lbm2d* h);
 
+/* ---- batched lattices (no reference counterpart: Orrery runs one lattice per solver) --------- */
+/* Many resident D3Q19 lattices of uneven size advanced together: each launch sweeps every
+ * lattice of one storage format, and `sample_every` steps plus a force sample per lattice are
+ * captured once into a hipGraph and replayed. Lattices are admitted against an HBM budget and
+ * keep their own step count, inlet and history; a lattice's results do not depend on the
+ * other lattices in the batch. */
+typedef struct lbm_batch lbm_batch;
+/* budget_bytes 0 means no budget; sample_every >= 1 steps between force samples. */
+int lbm_batch_create(size_t budget_bytes, uint32_t sample_every, lbm_batch** out);
+int lbm_batch_destroy(lbm_batch* b);
+/* Device bytes lbm_batch_add / lbm3d_create would allocate for this lattice. */
+int lbm_batch_required_bytes(const lbm3d_config* config, const uint8_t* solid, const double* distance,
+                             int32_t storage, size_t* bytes);
+/* Admits a lattice (LBM_ENOMEM before allocating when it exceeds the free budget), started at
+ * the uniform inlet state, perturbed by lbm_perturbation(seed, ...) * amplitude when
+ * amplitude != 0. */
+int lbm_batch_add(lbm_batch* b, const lbm3d_config* config, const uint8_t* solid, const double* distance,
+                  int32_t storage, uint64_t seed, double amplitude, int32_t* id);
+int lbm_batch_remove(lbm_batch* b, int32_t id);
+/* Advances every resident lattice by `steps` (a multiple of sample_every), sampling the force
+ * of each every sample_every steps; returns when the device is done. */
+int lbm_batch_run(lbm_batch* b, uint64_t steps);
+int lbm_batch_count(const lbm_batch* b, size_t* count);
+/* The lattice's handle (owned by the batch): lbm3d_force/_macroscopic/_populations/_steps
+ * work on it between runs. */
+int lbm_batch_lattice(lbm_batch* b, int32_t id, lbm3d** h);
+/* Force samples of a lattice: time (step count) and 6 doubles each (force xyz, moment xyz). */
+int lbm_batch_history(lbm_batch* b, int32_t id, double* time, double* force, size_t capacity,
+                      size_t* count);
+int lbm_batch_bytes(const lbm_batch* b, size_t* used, size_t* budget);
+/* Launch shape of the bulk kernel for the lattices of one storage format (autotuner). */
+int lbm_batch_set_launch(lbm_batch* b, int32_t storage, int32_t block, int32_t cells_per_thread);
+/* set_inlet_velocity of one lattice of the batch. */
+int lbm_batch_set_inlet(lbm_batch* b, int32_t id, const double velocity[3]);
+
 /* ---- host-only setup (no device needed) --------------------------------------------------- */
 /* validate() (src/aero/lbm3d/lbm3d_common.cpp:159-161): LBM_OK, or LBM_EINVAL with the
  * reference's reason in lbm_last_error(). */
@@ -206,6 +241,45 @@ int lbm_steady_since(const double* time, const double* drag, const double* lift,
 int lbm_settled_oscillation(const double* time, const double* drag, const double* lift, size_t n,
                             double from, double max_period_spread, double stationarity,
                             lbm_oscillation* out);
+
+/* ---- geometry: meshes, voxelisation, signed distance (src/geom) --------------------------- */
+/* Host-only, deterministic. Winding numbers follow MeshTree (winding_number.cpp: Barill et al.
+ * 2018 dipole far field, beta 2, leaves of 8 triangles); masks and distances are bit-identical
+ * to the reference's for the same mesh and grid. Errors: lbm_geom_last_error(). */
+typedef struct lbm_mesh lbm_mesh;
+const char* lbm_geom_last_error(void);
+/* TriangleMesh from vertices [nv][3] and triangles [nt][3] (triangle_mesh.h). */
+int lbm_mesh_create(const double* vertices, size_t nv, const uint32_t* triangles, size_t nt, lbm_mesh** out);
+int lbm_mesh_destroy(lbm_mesh* m);
+int lbm_mesh_size(const lbm_mesh* m, size_t* nv, size_t* nt);
+int lbm_mesh_data(const lbm_mesh* m, double* vertices, uint32_t* triangles);
+/* RevolutionProfile::mesh (primitives.cpp:312-379) of sphere_profile (kind 0: a radius),
+ * cylinder_profile (1: a radius, b height), cone_profile (2: a radius, b height) or
+ * capsule_profile (3: a max diameter, b nose radius, c shoulder radius, d afterbody half
+ * angle, e top diameter), with RevolutionResolution{around, arc_step}. */
+int lbm_mesh_revolution(int32_t kind, double a, double b, double c, double d, double e, int32_t around,
+                        double arc_step, lbm_mesh** out);
+/* Wedge::mesh (primitives.cpp:590-604). */
+int lbm_mesh_wedge(double length, double half_angle, double span, lbm_mesh** out);
+/* box_mesh (primitives.cpp:557-571), centred on the origin. */
+int lbm_mesh_box(double lx, double ly, double lz, lbm_mesh** out);
+/* naca_wing (naca.cpp:184-257) of Naca4{max_camber, camber_position, thickness}. */
+int lbm_mesh_naca_wing(double max_camber, double camber_position, double thickness, double chord, double span,
+                       int32_t n, int32_t span_divisions, lbm_mesh** out);
+/* v <- R (v - pivot) + translation, R row-major. */
+int lbm_mesh_transform(lbm_mesh* m, const double rotation[9], const double pivot[3], const double translation[3]);
+int lbm_mesh_bounds(const lbm_mesh* m, double lo[3], double hi[3]);
+/* voxelize (voxel_grid.cpp:81-102): cell (i,j,k) at index i + nx (j + ny k), centre
+ * origin + (i + 1/2, j + 1/2, k + 1/2) spacing; solid where winding >= threshold. */
+int lbm_voxelize(const lbm_mesh* m, const double origin[3], double spacing, uint64_t nx, uint64_t ny, uint64_t nz,
+                 double threshold, uint8_t* solid, double* winding);
+/* signed_distance_band (voxel_grid.cpp:104-138): distance to the surface, exact within
+ * band_cells * spacing, negative in solid cells; in_band optional. */
+int lbm_signed_distance_band(const lbm_mesh* m, const double origin[3], double spacing, uint64_t nx, uint64_t ny,
+                             uint64_t nz, const uint8_t* solid, double band_cells, double* distance,
+                             uint8_t* in_band);
+/* MeshTree::winding_number at n points [n][3]. */
+int lbm_winding_numbers(const lbm_mesh* m, const double* points, size_t n, double* out);
 
 /* ---- shared helpers ----------------------------------------------------------------------- */
 /* Seeded perturbation in [-1, 1): SplitMix64 of (seed, cell, component), 53-bit mantissa.

@@ -289,3 +289,115 @@ double ref_mean_since(const double* time, const double* values, std::size_t n, d
 }
 
 } // extern "C"
+
+#include "orrery/geom/naca.h"
+#include "orrery/geom/primitives.h"
+#include "orrery/geom/voxel_grid.h"
+#include "orrery/geom/winding_number.h"
+
+namespace {
+
+std::size_t copy_mesh(const orrery::geom::TriangleMesh& m, double* v, std::size_t vcap, std::uint32_t* t,
+                      std::size_t tcap, std::size_t* nt) {
+    for (std::size_t k = 0; k < m.vertices.size() && k < vcap; ++k) {
+        v[3 * k] = m.vertices[k].x;
+        v[3 * k + 1] = m.vertices[k].y;
+        v[3 * k + 2] = m.vertices[k].z;
+    }
+    for (std::size_t k = 0; k < m.triangles.size() && k < tcap; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            t[3 * k + a] = m.triangles[k][a];
+        }
+    }
+    *nt = m.triangles.size();
+    return m.vertices.size();
+}
+
+orrery::geom::TriangleMesh mesh_of(const double* v, std::size_t nv, const std::uint32_t* t, std::size_t nt) {
+    orrery::geom::TriangleMesh m;
+    for (std::size_t k = 0; k < nv; ++k) {
+        m.vertices.push_back({v[3 * k], v[3 * k + 1], v[3 * k + 2]});
+    }
+    for (std::size_t k = 0; k < nt; ++k) {
+        m.triangles.push_back({t[3 * k], t[3 * k + 1], t[3 * k + 2]});
+    }
+    return m;
+}
+
+} // namespace
+
+extern "C" {
+
+std::size_t ref_mesh_revolution(int kind, double a, double b, double c, double d, double e, int around,
+                                double arc_step, double* v, std::size_t vcap, std::uint32_t* t, std::size_t tcap,
+                                std::size_t* nt) {
+    using namespace orrery::geom;
+    RevolutionProfile p;
+    switch (kind) {
+        case 0: p = sphere_profile(a); break;
+        case 1: p = cylinder_profile(a, b); break;
+        case 2: p = cone_profile(a, b); break;
+        default: {
+            CapsuleShape s;
+            s.max_diameter = a;
+            s.nose_radius = b;
+            s.shoulder_radius = c;
+            s.afterbody_half_angle = d;
+            s.top_diameter = e;
+            p = capsule_profile(s);
+        }
+    }
+    return copy_mesh(p.mesh(RevolutionResolution{around, arc_step}), v, vcap, t, tcap, nt);
+}
+
+std::size_t ref_mesh_wedge(double length, double half_angle, double span, double* v, std::size_t vcap,
+                           std::uint32_t* t, std::size_t tcap, std::size_t* nt) {
+    orrery::geom::Wedge w;
+    w.length = length;
+    w.half_angle = half_angle;
+    w.span = span;
+    return copy_mesh(w.mesh(), v, vcap, t, tcap, nt);
+}
+
+std::size_t ref_mesh_naca_wing(double m, double p, double th, double chord, double span, int n, int div, double* v,
+                               std::size_t vcap, std::uint32_t* t, std::size_t tcap, std::size_t* nt) {
+    orrery::geom::Naca4 a;
+    a.max_camber = m;
+    a.camber_position = p;
+    a.thickness = th;
+    return copy_mesh(orrery::geom::naca_wing(a, chord, span, n, div).mesh, v, vcap, t, tcap, nt);
+}
+
+void ref_voxelize(const double* v, std::size_t nv, const std::uint32_t* t, std::size_t nt, const double* origin,
+                  double spacing, std::size_t nx, std::size_t ny, std::size_t nz, double threshold, std::uint8_t* solid,
+                  double* winding) {
+    using namespace orrery::geom;
+    const TriangleMesh m = mesh_of(v, nv, t, nt);
+    GridSpec g;
+    g.origin = {origin[0], origin[1], origin[2]};
+    g.spacing = spacing;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    const VoxelGrid vg = voxelize(m, g, threshold);
+    std::memcpy(solid, vg.solid.data(), vg.solid.size());
+    std::memcpy(winding, vg.winding.data(), vg.winding.size() * sizeof(double));
+}
+
+void ref_signed_distance_band(const double* v, std::size_t nv, const std::uint32_t* t, std::size_t nt,
+                              const double* origin, double spacing, std::size_t nx, std::size_t ny, std::size_t nz,
+                              const std::uint8_t* solid, double band_cells, double* distance) {
+    using namespace orrery::geom;
+    const TriangleMesh m = mesh_of(v, nv, t, nt);
+    VoxelGrid vg;
+    vg.spec.origin = {origin[0], origin[1], origin[2]};
+    vg.spec.spacing = spacing;
+    vg.spec.nx = nx;
+    vg.spec.ny = ny;
+    vg.spec.nz = nz;
+    vg.solid.assign(solid, solid + nx * ny * nz);
+    const SignedDistanceField f = signed_distance_band(m, vg, band_cells);
+    std::memcpy(distance, f.distance.data(), f.distance.size() * sizeof(double));
+}
+
+} // extern "C"

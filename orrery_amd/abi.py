@@ -105,6 +105,20 @@ EXPORTS = [
                                             c_sizep]),
     ("lbm2d_boundary_links", ctypes.c_int, [P(CConfig2D), c_u8p, c_u32p, c_u32p, ctypes.c_size_t, c_sizep]),
     ("lbm_perturbation", ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]),
+    ("lbm_batch_create", ctypes.c_int, [ctypes.c_size_t, ctypes.c_uint32, P(ctypes.c_void_p)]),
+    ("lbm_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm_batch_required_bytes", ctypes.c_int, [P(CConfig3D), c_u8p, c_f64p, ctypes.c_int32, c_sizep]),
+    ("lbm_batch_add", ctypes.c_int, [ctypes.c_void_p, P(CConfig3D), c_u8p, c_f64p, ctypes.c_int32, ctypes.c_uint64,
+                                     ctypes.c_double, P(ctypes.c_int32)]),
+    ("lbm_batch_remove", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("lbm_batch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    ("lbm_batch_count", ctypes.c_int, [ctypes.c_void_p, c_sizep]),
+    ("lbm_batch_lattice", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_void_p)]),
+    ("lbm_batch_history", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p, c_f64p, ctypes.c_size_t,
+                                         c_sizep]),
+    ("lbm_batch_bytes", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep]),
+    ("lbm_batch_set_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    ("lbm_batch_set_inlet", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p]),
     ("lbm_force_coefficients", None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, c_f64p,
                                       c_f64p]),
     ("lbm_mean_since", ctypes.c_double, [c_f64p, c_f64p, ctypes.c_size_t, ctypes.c_double]),
@@ -114,6 +128,26 @@ EXPORTS = [
                                         ctypes.c_double]),
     ("lbm_settled_oscillation", ctypes.c_int, [c_f64p, c_f64p, c_f64p, ctypes.c_size_t, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, P(COscillation)]),
+    ("lbm_geom_last_error", ctypes.c_char_p, []),
+    ("lbm_mesh_create", ctypes.c_int, [c_f64p, ctypes.c_size_t, P(ctypes.c_uint32), ctypes.c_size_t,
+                                       P(ctypes.c_void_p)]),
+    ("lbm_mesh_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm_mesh_size", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_size_t), P(ctypes.c_size_t)]),
+    ("lbm_mesh_data", ctypes.c_int, [ctypes.c_void_p, c_f64p, P(ctypes.c_uint32)]),
+    ("lbm_mesh_revolution", ctypes.c_int, [ctypes.c_int32] + [ctypes.c_double] * 5
+     + [ctypes.c_int32, ctypes.c_double, P(ctypes.c_void_p)]),
+    ("lbm_mesh_wedge", ctypes.c_int, [ctypes.c_double] * 3 + [P(ctypes.c_void_p)]),
+    ("lbm_mesh_box", ctypes.c_int, [ctypes.c_double] * 3 + [P(ctypes.c_void_p)]),
+    ("lbm_mesh_naca_wing", ctypes.c_int, [ctypes.c_double] * 5 + [ctypes.c_int32, ctypes.c_int32,
+                                                                  P(ctypes.c_void_p)]),
+    ("lbm_mesh_transform", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p]),
+    ("lbm_mesh_bounds", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p]),
+    ("lbm_voxelize", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64,
+                                    ctypes.c_uint64, ctypes.c_double, P(ctypes.c_uint8), c_f64p]),
+    ("lbm_signed_distance_band", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_double, ctypes.c_uint64,
+                                                ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint8),
+                                                ctypes.c_double, c_f64p, P(ctypes.c_uint8)]),
+    ("lbm_winding_numbers", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_size_t, c_f64p]),
 ]
 
 _lib = None

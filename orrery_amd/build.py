@@ -14,7 +14,13 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["lbm_setup.hip", "lbm3d_kernels.hip", "lbm3d_abi.hip", "lbm2d_kernels.hip", "lbm2d_abi.hip",
-           "lbm_batch.hip", "lbm_history.cpp"]
+           "lbm_batch.hip"]
+# Host-only sources, compiled by the host C++ compiler with the floating-point policy of the
+# reference's geometry and history code (-O2, no contraction, no fast-math) so that their FP64
+# decisions (voxel masks) match it bit for bit.
+HOST_SOURCES = ["geom.cpp", "lbm_history.cpp"]
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-pthread"]
 OUTPUT = os.path.join(HERE, "liborrery_lbm.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -32,8 +38,17 @@ def sources() -> list[str]:
 
 def build(verbose: bool = False, extra: list[str] | None = None) -> str:
     """Compiles the library; returns its path. Raises CalledProcessError on failure."""
+    objects = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(CSRC, "_build", src.replace(".cpp", ".o"))
+        os.makedirs(os.path.dirname(obj), exist_ok=True)
+        cmd = [CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objects.append(obj)
     tmp = OUTPUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, *(extra or []), "-o", tmp, *sources()]
+    cmd = [HIPCC, *FLAGS, *(extra or []), "-o", tmp, *objects, *sources(), "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -1,15 +1,9 @@
-// D3Q19 kernels: the fused in-place stream-collide step (bulk and boundary halves), state
-// initialisation, and the FP64 readbacks and reductions (populations, moments, mass, force).
-//
-// Physics follows the reference step (src/aero/lbm3d/lbm3d_cpu.cpp:32-100,261-322 and
-// src/render/shaders/lbm/lbm_d3q19_common.glsl:111-368): BGK with Guo forcing, the optional
-// Smagorinsky closure, absorbing layers, halfway or interpolated (Bouzidi) bounce-back,
-// slip / no-slip / moving walls, a regularised velocity inlet and a convective or
-// zero-gradient outlet. The binary32 formats collide on populations shifted by their weights
-// (the GLSL form); the binary64 format keeps the CPU solver's unshifted arithmetic so the
-// on-device FP64 run reproduces it.
+// D3Q19 kernels of one lattice: the fused in-place stream-collide step (bulk and boundary
+// halves, device functions in lbm3d_step.h), state initialisation, and the FP64 readbacks and
+// reductions (populations, moments, mass, force).
 #include <hip/hip_runtime.h>
 
+#include "lbm3d_step.h"
 #include "lbm_device.h"
 #include "lbm_kernels.h"
 
@@ -18,267 +12,14 @@ namespace lbm {
 namespace {
 
 using L3 = D3Q19;
+using d3::off3;
 
-__device__ __forceinline__ int off3(int i, int nx, int plane) {
-    return L3::cx(i) + nx * L3::cy(i) + plane * L3::cz(i);
-}
-
-// Relaxation rate of the absorbing layers at (x, y, z) (sponge_sigma,
-// src/aero/lbm3d/lbm3d_internal.h:256-281).
-template <typename R>
-__device__ R sponge_sigma3(const Lattice3<R>& p, std::uint32_t x, std::uint32_t y, std::uint32_t z) {
-    R best = 0;
-    const auto layer = [&best](std::uint32_t d, std::uint32_t t) {
-        if (d < t) {
-            const R r = static_cast<R>(t - d) / static_cast<R>(t);
-            best = best > r * r ? best : r * r;
-        }
-    };
-    layer(x, p.sponge_in);
-    layer(p.nx - 1 - x, p.sponge_out);
-    if (p.sponge_y) {
-        layer(y, p.sponge_sides);
-        layer(p.ny - 1 - y, p.sponge_sides);
-    }
-    if (p.sponge_z) {
-        layer(z, p.sponge_sides);
-        layer(p.nz - 1 - z, p.sponge_sides);
-    }
-    return p.sponge_strength * best;
-}
-
-// Collision of one node: f holds the post-streaming populations (shifted for binary32
-// formats) and is overwritten with the post-collision ones.
-template <class T>
-__device__ __forceinline__ void collide3(typename T::R (&f)[19], const Lattice3<typename T::R>& p,
-                                         typename T::R sponge) {
-    using R = typename T::R;
-    if constexpr (T::shifted) {
-        // lbm_d3q19_common.glsl:264-326 and :358-368 (shifted equilibrium w (drho + rho (...)))
-        R drho = 0;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            drho += f[i];
-        }
-        const R jx = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14];
-        const R jy = f[3] - f[4] + f[7] - f[8] - f[9] + f[10] + f[15] - f[16] + f[17] - f[18];
-        const R jz = f[5] - f[6] + f[11] - f[12] - f[13] + f[14] + f[15] - f[16] - f[17] + f[18];
-        const R rho = R(1) + drho;
-        const R inv = R(1) / rho;
-        const R ux = (jx + R(0.5) * p.force[0]) * inv;
-        const R uy = (jy + R(0.5) * p.force[1]) * inv;
-        const R uz = (jz + R(0.5) * p.force[2]) * inv;
-        const R uu = ux * ux + uy * uy + uz * uz;
-        R feq[19];
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            const R cu = R(L3::cx(i)) * ux + R(L3::cy(i)) * uy + R(L3::cz(i)) * uz;
-            feq[i] = R(L3::w(i)) * (drho + rho * (R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu));
-        }
-        R omega = p.omega;
-        if (p.les) {
-            R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
-#pragma unroll
-            for (int i = 0; i < 19; ++i) {
-                const R ex = R(L3::cx(i));
-                const R ey = R(L3::cy(i));
-                const R ez = R(L3::cz(i));
-                const R neq = f[i] - feq[i];
-                pxx += ex * ex * neq;
-                pyy += ey * ey * neq;
-                pzz += ez * ez * neq;
-                pxy += ex * ey * neq;
-                pxz += ex * ez * neq;
-                pyz += ey * ez * neq;
-            }
-            const R strain =
-                sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
-            const R tau = R(0.5) * (p.tau + sqrt(p.tau * p.tau + R(25.455844122715710) * p.smag_c2 * strain / rho));
-            omega = R(1) / tau;
-        }
-        const R force_scale = R(1) - R(0.5) * omega;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            R value = f[i] - omega * (f[i] - feq[i]);
-            if (p.forced) {
-                const R ex = R(L3::cx(i));
-                const R ey = R(L3::cy(i));
-                const R ez = R(L3::cz(i));
-                const R cu = ex * ux + ey * uy + ez * uz;
-                value += force_scale * R(L3::w(i)) *
-                         (R(3) * ((ex - ux) * p.force[0] + (ey - uy) * p.force[1] + (ez - uz) * p.force[2]) +
-                          R(9) * cu * (ex * p.force[0] + ey * p.force[1] + ez * p.force[2]));
-            }
-            f[i] = value;
-        }
-        if (sponge > R(0)) {
-            const R vx = p.inlet[0];
-            const R vy = p.inlet[1];
-            const R vz = p.inlet[2];
-            const R vv = vx * vx + vy * vy + vz * vz;
-#pragma unroll
-            for (int i = 0; i < 19; ++i) {
-                const R cu = R(L3::cx(i)) * vx + R(L3::cy(i)) * vy + R(L3::cz(i)) * vz;
-                const R target = R(L3::w(i)) * (R(3) * cu + R(4.5) * cu * cu - R(1.5) * vv);
-                f[i] -= sponge * (f[i] - target);
-            }
-        }
-    } else {
-        // lbm3d_cpu.cpp:32-100, expression for expression (the library is built without
-        // floating-point contraction).
-        double rho = 0.0;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            rho += f[i];
-        }
-        const double jx = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14];
-        const double jy = f[3] - f[4] + f[7] - f[8] - f[9] + f[10] + f[15] - f[16] + f[17] - f[18];
-        const double jz = f[5] - f[6] + f[11] - f[12] - f[13] + f[14] + f[15] - f[16] - f[17] + f[18];
-        const double fx = p.force[0];
-        const double fy = p.force[1];
-        const double fz = p.force[2];
-        const double ux = (jx + 0.5 * fx) / rho;
-        const double uy = (jy + 0.5 * fy) / rho;
-        const double uz = (jz + 0.5 * fz) / rho;
-        const double base = 1.0 - 1.5 * (ux * ux + uy * uy + uz * uz);
-        const double cu[19] = {0.0,     ux,       -ux,      uy,      -uy,     uz,       -uz,
-                               ux + uy, -ux - uy, ux - uy,  uy - ux, ux + uz, -ux - uz, ux - uz,
-                               uz - ux, uy + uz,  -uy - uz, uy - uz, uz - uy};
-        double feq[19];
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            feq[i] = L3::w(i) * rho * (base + cu[i] * (3.0 + 4.5 * cu[i]));
-        }
-        double tau = p.tau;
-        if (p.les) {
-            double pi[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int i = 0; i < 19; ++i) {
-                const double neq = f[i] - feq[i];
-                const double ex = L3::cx(i);
-                const double ey = L3::cy(i);
-                const double ez = L3::cz(i);
-                pi[0] += ex * ex * neq;
-                pi[1] += ey * ey * neq;
-                pi[2] += ez * ez * neq;
-                pi[3] += ex * ey * neq;
-                pi[4] += ex * ez * neq;
-                pi[5] += ey * ez * neq;
-            }
-            const double strain = sqrt(pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
-                                       2.0 * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]));
-            const double c = p.smag_c;
-            tau = 0.5 * (tau + sqrt(tau * tau + 18.0 * 1.4142135623730951 * c * c * strain / rho));
-        }
-        const double omega = 1.0 / tau;
-        const double force_scale = 1.0 - 0.5 * omega;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            double value = f[i] - omega * (f[i] - feq[i]);
-            if (p.forced) {
-                const double ex = L3::cx(i);
-                const double ey = L3::cy(i);
-                const double ez = L3::cz(i);
-                value += force_scale * L3::w(i) *
-                         (3.0 * ((ex - ux) * fx + (ey - uy) * fy + (ez - uz) * fz) +
-                          9.0 * cu[i] * (ex * fx + ey * fy + ez * fz));
-            }
-            f[i] = value;
-        }
-        if (sponge > 0.0) {
-            const double sx = p.inlet[0];
-            const double sy = p.inlet[1];
-            const double sz = p.inlet[2];
-#pragma unroll
-            for (int i = 0; i < 19; ++i) {
-                const double c = static_cast<double>(L3::cx(i)) * sx + static_cast<double>(L3::cy(i)) * sy +
-                                 static_cast<double>(L3::cz(i)) * sz;
-                const double vv = sx * sx + sy * sy + sz * sz;
-                const double target = L3::w(i) * 1.0 * (1.0 + 3.0 * c + 4.5 * c * c - 1.5 * vv);
-                f[i] -= sponge * (f[i] - target);
-            }
-        }
-    }
-}
-
-// Regularised velocity inlet on the west face (velocity_inlet_west,
-// src/aero/lbm3d/lbm3d_internal.h:324-365; shifted form lbm_d3q19_common.glsl:111-162).
-template <class T>
-__device__ __forceinline__ void inlet_west3(typename T::R (&f)[19], const Lattice3<typename T::R>& p) {
-    using R = typename T::R;
-    const R ux = p.inlet[0];
-    const R uy = p.inlet[1];
-    const R uz = p.inlet[2];
-    R known = 0;
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        if (L3::cx(i) == 0) {
-            known += f[i];
-        } else if (L3::cx(i) < 0) {
-            known += R(2) * f[i];
-        }
-    }
-    R rho;
-    R drho;
-    if constexpr (T::shifted) {
-        drho = (known + ux) / (R(1) - ux);
-        rho = R(1) + drho;
-    } else {
-        rho = known / (1.0 - ux);
-        drho = rho;
-    }
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        if (L3::cx(i) > 0) {
-            const R cu = R(L3::cx(i)) * ux + R(L3::cy(i)) * uy + R(L3::cz(i)) * uz;
-            f[i] = f[L3::opp(i)] + R(6) * R(L3::w(i)) * rho * cu;
-        }
-    }
-    const R uu = ux * ux + uy * uy + uz * uz;
-    R feq[19];
-    R pi[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        const R ex = R(L3::cx(i));
-        const R ey = R(L3::cy(i));
-        const R ez = R(L3::cz(i));
-        const R cu = ex * ux + ey * uy + ez * uz;
-        if constexpr (T::shifted) {
-            feq[i] = R(L3::w(i)) * (drho + rho * (R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu));
-        } else {
-            feq[i] = L3::w(i) * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-        }
-        const R neq = f[i] - feq[i];
-        pi[0] += ex * ex * neq;
-        pi[1] += ey * ey * neq;
-        pi[2] += ez * ez * neq;
-        pi[3] += ex * ey * neq;
-        pi[4] += ex * ez * neq;
-        pi[5] += ey * ez * neq;
-    }
-    const R cs2 = R(1.0 / 3.0);
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        const R ex = R(L3::cx(i));
-        const R ey = R(L3::cy(i));
-        const R ez = R(L3::cz(i));
-        f[i] = feq[i] + R(4.5) * R(L3::w(i)) *
-                            ((ex * ex - cs2) * pi[0] + (ey * ey - cs2) * pi[1] + (ez * ez - cs2) * pi[2] +
-                             R(2) * (ex * ey * pi[3] + ex * ez * pi[4] + ey * ez * pi[5]));
-    }
-}
-
-// ---- the step: bulk half ---------------------------------------------------------------------
-// Every bulk node (all 18 neighbours inside the domain and fluid) streams plainly: 19 loads,
-// the collision, 19 stores. kLocal selects the A step (odd n) or the B step (even n).
+// ---- the step kernels -------------------------------------------------------------------------
+// Bulk: every bulk node (all 18 neighbours inside the domain and fluid) streams plainly, CPT
+// cells per thread along x. Boundary: one thread per boundary node. kLocal selects the A step
+// (odd step count) or the B step (even).
 template <class T, bool kLocal, int CPT>
 __global__ __launch_bounds__(256) void d3q19_bulk(Lattice3<typename T::R> p) {
-    using S = typename T::S;
-    using R = typename T::R;
-    S* __restrict__ f = static_cast<S*>(p.f);
-    const std::uint32_t stride = p.stride;
-    const int nx = static_cast<int>(p.nx);
-    const int plane = static_cast<int>(p.nx * p.ny);
     const std::uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
@@ -286,165 +27,17 @@ __global__ __launch_bounds__(256) void d3q19_bulk(Lattice3<typename T::R> p) {
         if (cell >= p.cells) {
             return;
         }
-        if (p.cls[cell] != 1) {
-            continue;
-        }
-        R v[19];
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            if constexpr (kLocal) {
-                v[i] = T::load(f[i * stride + cell]);
-            } else {
-                v[i] = T::load(f[L3::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(cell) - off3(i, nx, plane))]);
-            }
-        }
-        R sponge = 0;
-        if (p.sponge_strength > R(0)) {
-            const std::uint32_t z = p.div_plane.div(cell);
-            const std::uint32_t rest = cell - z * static_cast<std::uint32_t>(plane);
-            const std::uint32_t y = p.div_nx.div(rest);
-            sponge = sponge_sigma3(p, rest - y * p.nx, y, z);
-        }
-        collide3<T>(v, p, sponge);
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            if constexpr (kLocal) {
-                f[L3::opp(i) * stride + cell] = T::pack(v[i]);
-            } else {
-                f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] = T::pack(v[i]);
-            }
+        if (p.cls[cell] == 1) {
+            d3::bulk_node<T, kLocal>(p, cell);
         }
     }
 }
 
-// ---- the step: boundary half -----------------------------------------------------------------
-// One thread per boundary node. Sources come from the precomputed codes; values other than
-// the node's own natural pull come from the stash of the previous step.
 template <class T, bool kLocal>
 __global__ __launch_bounds__(256) void d3q19_boundary(Lattice3<typename T::R> p) {
-    using S = typename T::S;
-    using R = typename T::R;
     const std::uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= p.nb) {
-        return;
-    }
-    S* __restrict__ f = static_cast<S*>(p.f);
-    const std::uint32_t stride = p.stride;
-    const std::uint32_t bs = p.bstride;
-    const int nx = static_cast<int>(p.nx);
-    const int plane = static_cast<int>(p.nx * p.ny);
-    const std::uint32_t cell = p.bcell[b];
-    const std::uint32_t z = p.div_plane.div(cell);
-    const std::uint32_t rest = cell - z * static_cast<std::uint32_t>(plane);
-    const std::uint32_t y = p.div_nx.div(rest);
-    const std::uint32_t x = rest - y * p.nx;
-    const R* __restrict__ st = p.stash + (kLocal ? 19U * bs : 0U);
-    R* __restrict__ sw = p.stash + (kLocal ? 0U : 19U * bs);
-
-    // Natural pull of direction i at this node (its own address in the A-A scheme).
-    const auto natural = [&](int i, std::uint32_t src) -> R {
-        if constexpr (kLocal) {
-            return T::load(f[i * stride + cell]);
-        } else {
-            return T::load(f[L3::opp(i) * stride + src]);
-        }
-    };
-
-    R v[19];
-    R wall_rho = R(-1);
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        const std::uint32_t code = p.bcode[i * bs + b];
-        const int kind = static_cast<int>(code & 7U);
-        const std::uint32_t payload = code >> kCodeShift;
-        if (kind == kNode) {
-            if ((code & (kCodeMirrorY | kCodeMirrorZ)) == 0U) {
-                v[i] = natural(i, payload);
-            } else {
-                int d = i;
-                if (code & kCodeMirrorY) d = L3::mirror_y(d);
-                if (code & kCodeMirrorZ) d = L3::mirror_z(d);
-                v[i] = st[d * bs + payload];
-            }
-        } else if (kind == kBounce) {
-            // reflected (lbm3d_internal.h:185-197)
-            const int o = L3::opp(i);
-            const R leaving = st[o * bs + b];
-            const R q = p.bq != nullptr ? p.bq[i * bs + b] : R(0.5);
-            if (q == R(0.5)) {
-                v[i] = leaving;
-            } else if (q < R(0.5)) {
-                const R farther = natural(o, static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane)));
-                v[i] = R(2) * q * leaving + (R(1) - R(2) * q) * farther;
-            } else {
-                v[i] = (leaving + (R(2) * q - R(1)) * st[i * bs + b]) / (R(2) * q);
-            }
-        } else if (kind == kWall) {
-            if (wall_rho < R(0)) {
-                R s = 0;
-#pragma unroll
-                for (int k = 0; k < 19; ++k) {
-                    s += st[k * bs + b];
-                }
-                wall_rho = T::shifted ? R(1) + s : s;
-            }
-            v[i] = st[L3::opp(i) * bs + b] + R(6) * R(L3::w(i)) * wall_rho * R(L3::cx(i)) * p.wall_u[payload];
-        } else if (kind == kInlet) {
-            v[i] = R(0);
-        } else {
-            int d = i;
-            if (code & kCodeMirrorY) d = L3::mirror_y(d);
-            if (code & kCodeMirrorZ) d = L3::mirror_z(d);
-            R up = st[d * bs + payload];
-            if (p.convective) {
-                R* h = p.hist + L3::outlet_slot(i) * bs + b;
-                up = (*h + p.outlet_speed * up) / (R(1) + p.outlet_speed);
-                *h = up;
-            }
-            v[i] = up;
-        }
-    }
-    if (p.x_mode == 1 && x == 0) {
-        inlet_west3<T>(v, p);
-    }
-    const R sponge = p.sponge_strength > R(0) ? sponge_sigma3(p, x, y, z) : R(0);
-    collide3<T>(v, p, sponge);
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        const S packed = T::pack(v[i]);
-        sw[i * bs + b] = T::load(packed);
-        if constexpr (kLocal) {
-            f[L3::opp(i) * stride + cell] = packed;
-        } else {
-            int tx = static_cast<int>(x) + L3::cx(i);
-            int ty = static_cast<int>(y) + L3::cy(i);
-            int tz = static_cast<int>(z) + L3::cz(i);
-            bool inside = true;
-            if (ty < 0 || ty >= static_cast<int>(p.ny)) {
-                if (p.side[ty < 0 ? 0 : 1] == kPeriodic) {
-                    ty = ty < 0 ? ty + static_cast<int>(p.ny) : ty - static_cast<int>(p.ny);
-                } else {
-                    inside = false;
-                }
-            }
-            if (tz < 0 || tz >= static_cast<int>(p.nz)) {
-                if (p.side[tz < 0 ? 2 : 3] == kPeriodic) {
-                    tz = tz < 0 ? tz + static_cast<int>(p.nz) : tz - static_cast<int>(p.nz);
-                } else {
-                    inside = false;
-                }
-            }
-            if (tx < 0 || tx >= nx) {
-                if (p.x_mode == 0) {
-                    tx = tx < 0 ? tx + nx : tx - nx;
-                } else {
-                    inside = false;
-                }
-            }
-            if (inside) {
-                f[i * stride + static_cast<std::uint32_t>(tx + nx * ty + plane * tz)] = packed;
-            }
-        }
+    if (b < p.nb) {
+        d3::boundary_node<T, kLocal>(p, b);
     }
 }
 

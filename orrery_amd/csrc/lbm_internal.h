@@ -1,0 +1,167 @@
+// Internals of the D3Q19 device handle shared by the single-lattice ABI (lbm3d_abi.hip) and
+// the batch engine (lbm_batch.hip): the handle itself, its device-memory accounting and the
+// kernel descriptor built from it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/orrery_lbm.h"
+#include "lbm_device.h"
+#include "lbm_host.h"
+#include "lbm_kernels.h"
+
+struct lbm3d {
+    lbm3d_config cfg{};
+    int storage = LBM_STORE_F32;
+    std::size_t cells = 0;
+    std::uint32_t stride = 0;
+    lbm::host::Tables tables;
+    std::vector<std::uint8_t> solid;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::uint64_t steps = 0;
+    double inlet[3] = {0, 0, 0};
+    int block = 256;
+    int cpt = 1;
+    // device memory
+    void* f = nullptr;
+    std::uint8_t* cls = nullptr;
+    std::uint32_t* bcell = nullptr;
+    std::uint32_t* bcode = nullptr;
+    void* bq = nullptr;
+    void* stash = nullptr;
+    void* hist = nullptr;
+    std::uint32_t* lcell = nullptr;
+    std::uint8_t* ldir = nullptr;
+    double* lq = nullptr;
+    std::uint32_t* lslot = nullptr;
+    double* partial = nullptr;
+    double* result = nullptr;
+    std::size_t device_bytes = 0;
+    std::vector<void*> owned;
+};
+
+namespace lbm::internal {
+
+inline std::size_t store_bytes(int storage) {
+    switch (storage) {
+        case LBM_STORE_F16S:
+        case LBM_STORE_BF16S:
+            return 2;
+        case LBM_STORE_F64:
+            return 8;
+        default:
+            return 4;
+    }
+}
+
+inline std::size_t real_bytes(int storage) { return storage == LBM_STORE_F64 ? 8 : 4; }
+
+// Device bytes of a lattice with these tables (the admission rule of the batch engine and of
+// the memory check; the counterpart of gpu_memory_bytes, lbm3d_common.cpp:276-300).
+inline std::size_t lattice_bytes(std::size_t cells, std::uint32_t stride, const lbm::host::Tables& t, int storage,
+                                 bool interpolated) {
+    const std::size_t sb = store_bytes(storage);
+    const std::size_t rb = real_bytes(storage);
+    std::size_t total = 19U * static_cast<std::size_t>(stride) * sb; // populations (in place)
+    total += cells;                                                     // node classes (u8)
+    total += t.bcell.size() * 4U;                                       // boundary list
+    total += 19U * static_cast<std::size_t>(t.bstride) * 4U;            // source codes
+    if (interpolated) {
+        total += 19U * static_cast<std::size_t>(t.bstride) * rb; // wall fractions
+    }
+    total += 2U * 19U * static_cast<std::size_t>(t.bstride) * rb; // stash
+    total += 5U * static_cast<std::size_t>(t.bstride) * rb;       // outlet history
+    total += t.links.size() * (4U + 1U + 8U + 4U);                 // links
+    const std::size_t red = std::max<std::size_t>(lbm::reduction_blocks(cells) + lbm::reduction_blocks(t.bcell.size()),
+                                                  6U * lbm::reduction_blocks(t.links.size()));
+    total += (red + 8U) * 8U;
+    return total;
+}
+
+// Bytes lbm3d_create would allocate for this configuration and mask.
+inline std::size_t required_bytes3(const lbm3d_config& c, const std::uint8_t* solid, const double* distance,
+                                   int storage) {
+    const std::size_t cells = c.nx * c.ny * c.nz;
+    const lbm::host::Tables t = lbm::host::tables3(c, solid, distance);
+    return lattice_bytes(cells, lbm::host::padded(cells, 128), t, storage, c.solid_wall == LBM_SOLID_INTERPOLATED);
+}
+
+template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
+    using R = typename T::R;
+    lbm::Lattice3<R> p{};
+    const lbm3d_config& c = h.cfg;
+    p.f = h.f;
+    p.cls = h.cls;
+    p.bcell = h.bcell;
+    p.bcode = h.bcode;
+    p.bq = static_cast<const R*>(h.bq);
+    p.stash = static_cast<R*>(h.stash);
+    p.hist = static_cast<R*>(h.hist);
+    p.nx = static_cast<std::uint32_t>(c.nx);
+    p.ny = static_cast<std::uint32_t>(c.ny);
+    p.nz = static_cast<std::uint32_t>(c.nz);
+    p.cells = static_cast<std::uint32_t>(h.cells);
+    p.stride = h.stride;
+    p.nb = static_cast<std::uint32_t>(h.tables.bcell.size());
+    p.bstride = h.tables.bstride;
+    p.div_nx = lbm::FastDiv(p.nx);
+    p.div_plane = lbm::FastDiv(p.nx * p.ny);
+    p.x_mode = c.x_boundary;
+    p.convective = c.outlet == LBM_OUTLET_CONVECTIVE ? 1 : 0;
+    p.side[0] = c.y_low.kind;
+    p.side[1] = c.y_high.kind;
+    p.side[2] = c.z_low.kind;
+    p.side[3] = c.z_high.kind;
+    p.wall_u[0] = static_cast<R>(c.y_low.velocity_x);
+    p.wall_u[1] = static_cast<R>(c.y_high.velocity_x);
+    p.wall_u[2] = static_cast<R>(c.z_low.velocity_x);
+    p.wall_u[3] = static_cast<R>(c.z_high.velocity_x);
+    for (int a = 0; a < 3; ++a) {
+        p.inlet[a] = static_cast<R>(h.inlet[a]);
+    }
+    p.tau = static_cast<R>(c.tau);
+    p.omega = static_cast<R>(1) / static_cast<R>(c.tau);
+    const double k = c.smagorinsky_enabled ? c.smagorinsky_constant : 0.0;
+    p.smag_c = static_cast<R>(k);
+    p.smag_c2 = static_cast<R>(k * k);
+    // The binary32 kernels switch the closure off when C^2 is 0, as the GLSL does
+    // (lbm_d3q19_common.glsl:287); the binary64 one follows the CPU's flag.
+    p.les = T::shifted ? (p.smag_c2 > R(0) ? 1 : 0) : (c.smagorinsky_enabled ? 1 : 0);
+    p.force[0] = static_cast<R>(c.body_force_x);
+    p.force[1] = static_cast<R>(c.body_force_y);
+    p.force[2] = static_cast<R>(c.body_force_z);
+    p.forced = (c.body_force_x != 0.0 || c.body_force_y != 0.0 || c.body_force_z != 0.0) ? 1 : 0;
+    p.outlet_speed = static_cast<R>(c.inlet_ux);
+    const bool sponge = c.sponge_strength > 0.0 && c.x_boundary == LBM_X_INFLOW_OUTFLOW;
+    p.sponge_strength = sponge ? static_cast<R>(c.sponge_strength) : R(0);
+    p.sponge_in = static_cast<std::uint32_t>(c.sponge_inlet);
+    p.sponge_out = static_cast<std::uint32_t>(c.sponge_outlet);
+    p.sponge_sides = static_cast<std::uint32_t>(c.sponge_sides);
+    p.sponge_y = c.y_low.kind != LBM_WALL_PERIODIC ? 1 : 0;
+    p.sponge_z = c.z_low.kind != LBM_WALL_PERIODIC ? 1 : 0;
+    return p;
+}
+
+struct LinkView {
+    const std::uint32_t* cell;
+    const std::uint8_t* dir;
+    const double* q;
+    const std::uint32_t* slot;
+    std::uint32_t count;
+};
+
+inline LinkView link_view(const lbm3d& h) {
+    return LinkView{h.lcell, h.ldir, h.lq, h.lslot, static_cast<std::uint32_t>(h.tables.links.size())};
+}
+
+inline int storage_of(const lbm3d& h) { return h.storage; }
+
+// The batch engine advanced the lattice on its own stream.
+inline void advance_steps(lbm3d& h, std::uint64_t steps) { h.steps += steps; }
+
+} // namespace lbm::internal

@@ -12,7 +12,7 @@ import dataclasses
 import numpy as np
 
 MAX_GRID_CELLS = 1 << 27  # geom::max_grid_cells (src/geom/include/orrery/geom/voxel_grid.h:29)
-FAR_DISTANCE = 1e9        # far_distance (solid_mask3d.h:247)
+FAR_DISTANCE = 1e9        # far_distance (solid_mask3d.h:82)
 
 
 @dataclasses.dataclass

@@ -52,6 +52,19 @@ def load():
         "ref2d_zou_he_west": (None, [_f64p, ctypes.c_double, ctypes.c_double]),
         "ref2d_circle_mask": (None, [u64, u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u8p]),
         "ref_lattice_tau": (ctypes.c_double, [ctypes.c_double] * 6),
+        "ref_mesh_revolution": (ctypes.c_size_t, [ctypes.c_int] + [ctypes.c_double] * 5
+                                + [ctypes.c_int, ctypes.c_double, _f64p, ctypes.c_size_t, _u32p, ctypes.c_size_t,
+                                   P(ctypes.c_size_t)]),
+        "ref_mesh_wedge": (ctypes.c_size_t, [ctypes.c_double] * 3 + [_f64p, ctypes.c_size_t, _u32p, ctypes.c_size_t,
+                                                                     P(ctypes.c_size_t)]),
+        "ref_mesh_naca_wing": (ctypes.c_size_t, [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_int, _f64p,
+                                                                         ctypes.c_size_t, _u32p, ctypes.c_size_t,
+                                                                         P(ctypes.c_size_t)]),
+        "ref_voxelize": (None, [_f64p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _f64p, ctypes.c_double, ctypes.c_size_t,
+                                ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double, _u8p, _f64p]),
+        "ref_signed_distance_band": (None, [_f64p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _f64p, ctypes.c_double,
+                                            ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _u8p, ctypes.c_double,
+                                            _f64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -131,3 +144,50 @@ def boundary_links2(config, mask):
     lib.ref2d_boundary_links(ctypes.byref(c), ptr(solid, ctypes.c_uint8), ptr(cell, ctypes.c_uint32),
                              ptr(d, ctypes.c_uint32), n)
     return cell, d
+
+
+def _mesh(call):
+    """Runs a ref_mesh_* binding twice: once for the sizes, once into arrays of that size."""
+    nt = ctypes.c_size_t()
+    nv = call(None, 0, None, 0, ctypes.byref(nt))
+    v = np.empty((nv, 3))
+    t = np.empty((nt.value, 3), dtype=np.uint32)
+    call(ptr(v), nv, ptr(t, ctypes.c_uint32), nt.value, ctypes.byref(nt))
+    return v, t
+
+
+def revolution_mesh(kind, a, b=0.0, c=0.0, d=0.0, e=0.0, around=64, arc_step=0.05):
+    lib = load()
+    return _mesh(lambda *r: lib.ref_mesh_revolution(kind, a, b, c, d, e, around, arc_step, *r))
+
+
+def wedge_mesh(length, half_angle, span):
+    lib = load()
+    return _mesh(lambda *r: lib.ref_mesh_wedge(length, half_angle, span, *r))
+
+
+def naca_wing(m, p, t, chord, span, n, div):
+    lib = load()
+    return _mesh(lambda *r: lib.ref_mesh_naca_wing(m, p, t, chord, span, n, div, *r))
+
+
+def voxelize(v, t, origin, spacing, nx, ny, nz, threshold=0.5):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.uint32)
+    o = np.asarray(origin, dtype=np.float64)
+    solid = np.empty(nx * ny * nz, dtype=np.uint8)
+    w = np.empty(nx * ny * nz)
+    load().ref_voxelize(ptr(v), len(v), ptr(t, ctypes.c_uint32), len(t), ptr(o), spacing, nx, ny, nz, threshold,
+                        ptr(solid, ctypes.c_uint8), ptr(w))
+    return solid, w
+
+
+def signed_distance_band(v, t, origin, spacing, nx, ny, nz, solid, band_cells=3.0):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.uint32)
+    o = np.asarray(origin, dtype=np.float64)
+    d = np.empty(nx * ny * nz)
+    load().ref_signed_distance_band(ptr(v), len(v), ptr(t, ctypes.c_uint32), len(t), ptr(o), spacing, nx, ny, nz,
+                                    ptr(np.ascontiguousarray(solid, dtype=np.uint8), ctypes.c_uint8), band_cells,
+                                    ptr(d))
+    return d

@@ -1,0 +1,76 @@
+"""Batched lattices on the device: every lattice of a batch evolves exactly as the same lattice
+alone (bitwise, in every storage format), its force samples equal the single-lattice force, and
+admission refuses lattices beyond the HBM budget before allocating."""
+import numpy as np
+import pytest
+
+from cases import sphere_channel
+from orrery_amd import Lbm3DSolver, LbmError, Smagorinsky, SolidWall
+from orrery_amd.batch import LatticeBatch
+
+pytestmark = pytest.mark.gpu
+
+
+def lattices():
+    a, ma = sphere_channel(48, 24, 24, 8.0, 2.0, 0.05, 40.0)
+    b, mb = sphere_channel(64, 20, 28, 6.0, 3.0, 0.04, 80.0)
+    b.smagorinsky = Smagorinsky(True, 0.17)
+    c, mc = sphere_channel(40, 30, 18, 7.0, 2.0, 0.05, 30.0)
+    c.solid_wall = SolidWall.interpolated
+    return [(a, ma, "f32", 11), (b, mb, "f32", 12), (c, mc, "f16s", 13), (a, ma, "bf16s", 14)]
+
+
+def alone(config, mask, storage, seed, steps):
+    s = Lbm3DSolver(config, mask, storage)
+    s.initialize_perturbed(seed, 1e-4)
+    forces = []
+    for _ in range(steps // 10):
+        s.step(10)
+        forces.append(s.force())
+    return s, forces
+
+
+def test_batch_matches_single_lattices_bitwise():
+    steps = 60
+    batch = LatticeBatch(sample_every=10)
+    members = [(batch.add(c, m, st, seed, 1e-4), c, m, st, seed) for c, m, st, seed in lattices()]
+    batch.run(30)
+    batch.run(30)
+    for lat, c, m, st, seed in members:
+        print(f"lattice {lat.id} {c.nx}x{c.ny}x{c.nz} {st}", flush=True)
+        single, forces = alone(c, m, st, seed, steps)
+        assert lat.steps() == steps
+        assert np.array_equal(lat.populations(), single.populations())
+        t, f = lat.history()
+        assert t.tolist() == [10.0 * (k + 1) for k in range(steps // 10)]
+        for k, fs in enumerate(forces):
+            assert tuple(f[k, :3]) == fs.force and tuple(f[k, 3:]) == fs.moment
+
+
+def test_a_lattice_is_the_same_alone_and_in_the_mix():
+    c, m, st, seed = lattices()[1]
+    solo = LatticeBatch(sample_every=10)
+    x = solo.add(c, m, st, seed, 1e-4)
+    solo.run(40)
+    mix = LatticeBatch(sample_every=10)
+    others = [mix.add(cc, mm, ss, sd, 1e-4) for cc, mm, ss, sd in lattices()]
+    mix.run(20)
+    y = mix.add(c, m, st, seed, 1e-4)  # admitted while the others are running
+    mix.remove(others[0])
+    mix.run(40)
+    assert np.array_equal(x.populations(), y.populations())
+    tx, fx = x.history()
+    ty, fy = y.history()
+    assert np.array_equal(tx, ty) and np.array_equal(fx, fy)
+
+
+def test_admission_refuses_beyond_the_budget():
+    c, m, st, seed = lattices()[0]
+    need = LatticeBatch.required_bytes(c, m, "f32")
+    batch = LatticeBatch(budget_bytes=2 * need + need // 2)
+    batch.add(c, m, "f32")
+    batch.add(c, m, "f32")
+    with pytest.raises(LbmError, match="of the budget are free"):
+        batch.add(c, m, "f32")
+    used, budget = batch.bytes()
+    assert used == 2 * need
