@@ -90,6 +90,32 @@ def cpu_baseline(config, mask, steps):
                       f"step ({dt:.1f} s), OpenMP over rows, host CPU {platform.processor() or platform.machine()}"}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB (torch's elementwise copy kernel), three times."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"bench: copied {mib} MiB three times (calibration kernel)", file=sys.stderr, flush=True)
+    del src, dst
+    torch.cuda.empty_cache()
+
+
+TRAFFIC = os.path.join(ROOT, "profiles", "d3q19_bulk_f32_512x256x256.traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the f32 bulk kernel on this workload, from the two counter runs
+    (FETCH_SIZE, WRITE_SIZE) summarised in profiles/, scaled by the copy-kernel calibration."""
+    if not os.path.exists(TRAFFIC):
+        return None
+    with open(TRAFFIC) as f:
+        return json.load(f)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +123,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--storages", default="f32,f16s,bf16s")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device with a plain copy kernel (3 launches), the "
+                         "known-size reference for the HBM byte counters of a profiler run")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -104,6 +133,8 @@ def main(argv=None):
               file=sys.stderr)
         return 2
 
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
     config, mask = workload()
     per_kernel_steps = min(200, max(20, args.steps // 5))
     results = {}
@@ -116,8 +147,11 @@ def main(argv=None):
     glups = cells / (main_r["ms_per_step"] * 1e-3) / 1e9
     bytes_per_node = 2 * 19 * 4
     achieved = bytes_per_node * main_r["bulk_nodes"] / (main_r["bulk_ms"] * 1e-3)
-    print("bench: timing the CPU baseline (oracle)", file=sys.stderr, flush=True)
-    cpu = cpu_baseline(config, mask, args.cpu_steps)
+    cpu = None
+    if args.cpu_steps > 0:
+        print("bench: timing the CPU baseline (oracle)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(config, mask, args.cpu_steps)
+    traffic = measured_traffic()
     line = {
         "metric": "D3Q19 GLUPS (FP32 A-A, BGK+Smagorinsky) and % HBM roofline",
         "value": round(glups, 3),
@@ -147,7 +181,9 @@ def main(argv=None):
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
-            "traffic": None,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "traffic_over_algorithmic": round(traffic["bytes_per_launch"] / (bytes_per_node * main_r["bulk_nodes"]), 4)
+            if traffic else None,
             "kernel": "d3q19_bulk<StoreF32>",
             "bulk_ms": round(main_r["bulk_ms"], 4),
             "boundary_ms": round(main_r["boundary_ms"], 4),

@@ -37,11 +37,29 @@ def test_poiseuille_and_couette_agree():
     assert v < 1e-4
 
 
-@pytest.mark.parametrize("outlet", [Outlet.extrapolation, Outlet.convective])
-def test_cylinder_channel_f32_before_shedding(outlet):
-    c, m = cylinder_channel_2d(outlet=outlet)
-    v, p, fx, _, _ = compare(c, m, 2000, "f32")
+@pytest.mark.parametrize("les", [False, True])
+def test_cylinder_channel_f32(les):
+    """The reference's case (test_lbm_gpu.cpp:259-276): 200 x 100, D 10 at (50, 50.5), Re 100,
+    4000 steps, with and without the closure."""
+    c, m = cylinder_channel_2d(200, 100, 50.0, 50.5, 10.0, 0.05, 100.0)
+    if les:
+        c.smagorinsky = Smagorinsky(True, 0.17)
+    v, p, fx, _, _ = compare(c, m, 4000, "f32")
     assert v < 1e-4 and p < 1e-4 and fx < 1e-4
+
+
+@pytest.mark.parametrize("outlet", [Outlet.extrapolation, Outlet.convective])
+def test_plain_channel_f32_force_coefficient(outlet):
+    """400 x 160, D 32: the drag swings through zero while the start-up waves cross the
+    channel, so the force is compared on the coefficient scale F / (u^2 D / 2)."""
+    c, m = cylinder_channel_2d(outlet=outlet)
+    v, p, _, cpu, gpu = compare(c, m, 2000, "f32")
+    scale = 0.5 * 0.05 ** 2 * 32.0
+    fc, fg = cpu.force(), gpu.force()
+    dcd = abs(fg.x - fc.x) / scale
+    dcl = abs(fg.y - fc.y) / scale
+    print(f"measured: D2Q9 f32 400x160 {outlet.name}: |dCd| {dcd:.3e} |dCl| {dcl:.3e}")
+    assert v < 1e-4 and p < 1e-4 and dcd < 1e-4 and dcl < 1e-4
 
 
 @pytest.mark.parametrize("outlet", [Outlet.extrapolation, Outlet.convective])
