@@ -138,9 +138,17 @@ int lbm3d_links(const lbm3d* h, uint32_t* cell, uint32_t* direction, double* q, 
 int lbm3d_device_bytes(const lbm3d* h, size_t* bytes);
 /* Blocks until the handle's stream has drained (the reference's finish()). */
 int lbm3d_synchronize(lbm3d* h);
-/* Launch shape of the step kernel: block size (64..256, multiple of 64) and cells per
- * thread (1, 2 or 4). 0 keeps the current value. Used by the autotuner. */
+/* Launch shape of the bulk kernel: block size (64..256, multiple of 64) and cells per thread
+ * (1, 2 or 4 consecutive cells, each population plane moved as one access). 0 keeps the
+ * current value. Used by the autotuner. */
 int lbm3d_set_launch(lbm3d* h, int32_t block, int32_t cells_per_thread);
+/* How a step is launched: LBM_SCHEDULE_SERIAL (default) runs the bulk kernel then the
+ * boundary kernel on the handle's stream; LBM_SCHEDULE_STREAMS runs the two concurrently on
+ * two streams; LBM_SCHEDULE_FUSED runs bulk and boundary nodes in one kernel in cell order.
+ * The launch shape of lbm3d_set_launch applies to the bulk kernel of the first two. All three
+ * give the same results bit for bit. */
+enum { LBM_SCHEDULE_FUSED = 0, LBM_SCHEDULE_SERIAL = 1, LBM_SCHEDULE_STREAMS = 2 };
+int lbm3d_set_schedule(lbm3d* h, int32_t schedule);
 /* Bytes one step of this lattice moves by the algorithmic count: 2 * 19 * bytes per stored
  * population, times nx * ny * nz (the reference's MLUPS convention). */
 int lbm3d_step_bytes(const lbm3d* h, double* bytes);

@@ -14,6 +14,7 @@ LIBRARY = os.path.join(HERE, "liborrery_lbm.so")
 
 LBM_OK, LBM_EINVAL, LBM_ENOMEM, LBM_EDEVICE, LBM_ESTATE = 0, 1, 2, 3, 4
 STORAGE = {"f32": 0, "f16s": 1, "bf16s": 2, "f64": 3}
+SCHEDULE = {"fused": 0, "serial": 1, "streams": 2}
 STORAGE_BYTES = {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}
 
 
@@ -82,6 +83,7 @@ EXPORTS = [
     ("lbm3d_links", ctypes.c_int, [ctypes.c_void_p, c_u32p, c_u32p, c_f64p, ctypes.c_size_t, c_sizep]),
     ("lbm3d_device_bytes", ctypes.c_int, [ctypes.c_void_p, c_sizep]),
     ("lbm3d_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm3d_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("lbm3d_set_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("lbm3d_step_bytes", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_counts", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep, c_sizep, c_sizep]),

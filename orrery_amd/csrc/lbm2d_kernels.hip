@@ -232,7 +232,7 @@ __global__ __launch_bounds__(256) void d2q9_boundary(Lattice2<typename T::R> p) 
                 }
                 wall_rho = T::shifted ? R(1) + s : s;
             }
-            v[i] = st[L2::opp(i) * bs + b] + R(6) * R(L2::w(i)) * wall_rho * R(L2::cx(i)) * p.wall_u[payload];
+            v[i] = st[L2::opp(i) * bs + b] + R(6) * R(L2::w(i)) * wall_rho * R(L2::cx(i)) * (payload == 0U ? p.wall_u[0] : p.wall_u[1]);
         } else if (kind == kInlet) {
             v[i] = R(0);
         } else {
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void d2q9_boundary(Lattice2<typename T::R> p) 
             int ty = static_cast<int>(y) + L2::cy(i);
             bool inside = true;
             if (ty < 0 || ty >= static_cast<int>(p.ny)) {
-                if (p.side[ty < 0 ? 0 : 1] == kPeriodic) {
+                if ((ty < 0 ? p.side[0] : p.side[1]) == kPeriodic) {
                     ty = ty < 0 ? ty + static_cast<int>(p.ny) : ty - static_cast<int>(p.ny);
                 } else {
                     inside = false;

@@ -76,10 +76,38 @@ void release(lbm3d* h) {
     for (void* p : h->owned) {
         (void)hipFree(p);
     }
+    if (h->side != nullptr) {
+        (void)hipStreamSynchronize(h->side);
+        (void)hipStreamDestroy(h->side);
+    }
+    if (h->fork != nullptr) {
+        (void)hipEventDestroy(h->fork);
+    }
+    if (h->join != nullptr) {
+        (void)hipEventDestroy(h->join);
+    }
     if (h->stream != nullptr) {
         (void)hipStreamDestroy(h->stream);
     }
     delete h;
+}
+
+// One step under the handle's schedule: one fused launch; the bulk and boundary kernels one
+// after the other; or the two kernels concurrently on two streams.
+template <class T> hipError_t step_once(lbm3d& h, const lbm::Lattice3<typename T::R>& p, bool local) {
+    if (h.schedule == LBM_SCHEDULE_FUSED) {
+        return lbm::d3q19_step_fused<T>(p, local, h.stream);
+    }
+    if (h.schedule == LBM_SCHEDULE_SERIAL || p.nb == 0) {
+        return lbm::d3q19_step<T>(p, local, h.block, h.cpt, h.stream);
+    }
+    hipError_t e = hipEventRecord(h.fork, h.stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h.side, h.fork, 0);
+    if (e == hipSuccess) e = lbm::d3q19_step_boundary<T>(p, local, h.side);
+    if (e == hipSuccess) e = lbm::d3q19_step_bulk<T>(p, local, h.block, h.cpt, h.stream);
+    if (e == hipSuccess) e = hipEventRecord(h.join, h.side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h.stream, h.join, 0);
+    return e;
 }
 
 int sync(lbm3d& h, const char* what) {
@@ -149,7 +177,8 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     const std::uint32_t stride = lbm::host::padded(cells, 128);
     // Every index the kernels form (population i * stride + cell) must fit in 32 bits
     // (lbm3d_gpu.cpp:492-497).
-    if (static_cast<std::uint64_t>(stride) * 19U > std::numeric_limits<std::uint32_t>::max()) {
+    if (static_cast<std::uint64_t>(stride) * 19U + 2U * lbm::internal::guard_cells(c.nx, c.ny) >
+        std::numeric_limits<std::uint32_t>::max()) {
         return fail(LBM_EINVAL, "lbm3d gpu: lattice too large for 32-bit indices");
     }
     if (storage != LBM_STORE_F64) {
@@ -169,7 +198,8 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     h->tables = lbm::host::tables3(c, solid, distance);
     const bool interpolated = c.solid_wall == LBM_SOLID_INTERPOLATED;
     // check_memory (lbm3d_gpu.cpp:131-149): refuse before allocating.
-    const std::size_t need = lattice_bytes(cells, stride, h->tables, storage, interpolated);
+    const std::size_t guard = lbm::internal::guard_cells(c.nx, c.ny);
+    const std::size_t need = lattice_bytes(cells, stride, guard, h->tables, storage, interpolated);
     std::size_t available = 0;
     if (int rc = lbm::host::device_memory(&available, nullptr); rc != LBM_OK) {
         delete h;
@@ -185,8 +215,17 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     if (e == hipSuccess) {
         e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     }
+    if (e == hipSuccess) {
+        e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
+    }
+    if (e == hipSuccess) {
+        e = hipEventCreateWithFlags(&h->fork, hipEventDisableTiming);
+    }
+    if (e == hipSuccess) {
+        e = hipEventCreateWithFlags(&h->join, hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
-        delete h;
+        release(h);
         return hip_fail("stream", e);
     }
     const lbm::host::Tables& t = h->tables;
@@ -198,10 +237,13 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
                                                   6U * lbm::reduction_blocks(nl));
     int rc = LBM_OK;
     void* tmp = nullptr;
-    if ((rc = alloc(*h, &h->f, 19U * static_cast<std::size_t>(stride) * sb, "population buffer")) ||
-        (rc = alloc(*h, &tmp, cells, "node class buffer")) || ((h->cls = static_cast<std::uint8_t*>(tmp)), false) ||
+    if ((rc = alloc(*h, &h->f, (19U * static_cast<std::size_t>(stride) + 2U * guard) * sb, "population buffer")) ||
+        ((h->f = static_cast<char*>(h->f) + guard * sb), false) ||
+        (rc = alloc(*h, &tmp, stride, "node class buffer")) || ((h->cls = static_cast<std::uint8_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, std::max<std::size_t>(nb, 1) * 4U, "boundary list")) ||
         ((h->bcell = static_cast<std::uint32_t*>(tmp)), false) ||
+        (rc = alloc(*h, &tmp, t.tile_slot.size() * 4U, "boundary slot table")) ||
+        ((h->tile_slot = static_cast<std::uint32_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, 19U * static_cast<std::size_t>(t.bstride) * 4U, "source codes")) ||
         ((h->bcode = static_cast<std::uint32_t*>(tmp)), false) ||
         (interpolated && (rc = alloc(*h, &h->bq, 19U * static_cast<std::size_t>(t.bstride) * rb, "wall buffer"))) ||
@@ -229,7 +271,17 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
         ldir[k] = static_cast<std::uint8_t>(t.links[k].direction);
         lq[k] = t.links[k].q;
     }
+    // The class array is padded to the plane stride with solid entries, so the bulk kernel
+    // reads the classes of a whole run of cells at once.
+    if (stride > cells) {
+        const hipError_t pe = hipMemsetAsync(h->cls + cells, 0, stride - cells, h->stream);
+        if (pe != hipSuccess) {
+            release(h);
+            return hip_fail("node class padding", pe);
+        }
+    }
     if ((rc = upload(*h, h->cls, t.cls, "node class upload")) || (rc = upload(*h, h->bcell, t.bcell, "boundary upload")) ||
+        (rc = upload(*h, h->tile_slot, t.tile_slot, "boundary slot upload")) ||
         (rc = upload(*h, h->bcode, t.bcode, "code upload")) || (rc = upload(*h, h->lcell, lcell, "link upload")) ||
         (rc = upload(*h, h->ldir, ldir, "link upload")) || (rc = upload(*h, h->lq, lq, "link upload")) ||
         (rc = upload(*h, h->lslot, t.lslot, "link upload"))) {
@@ -286,7 +338,7 @@ int lbm3d_step(lbm3d* h, uint64_t count) {
     for (std::uint64_t n = 0; n < count; ++n) {
         const bool local = (h->steps & 1U) != 0;
         const int rc = with_store(*h, [&]<class T>() -> int {
-            const hipError_t e = lbm::d3q19_step<T>(descriptor<T>(*h), local, h->block, h->cpt, h->stream);
+            const hipError_t e = step_once<T>(*h, descriptor<T>(*h), local);
             return e == hipSuccess ? ok() : hip_fail("step", e);
         });
         if (rc != LBM_OK) {
@@ -480,6 +532,17 @@ int lbm3d_set_launch(lbm3d* h, int32_t block, int32_t cells_per_thread) {
     return ok();
 }
 
+int lbm3d_set_schedule(lbm3d* h, int32_t schedule) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    if (schedule != LBM_SCHEDULE_FUSED && schedule != LBM_SCHEDULE_SERIAL && schedule != LBM_SCHEDULE_STREAMS) {
+        return fail(LBM_EINVAL, "lbm3d: unknown step schedule");
+    }
+    h->schedule = schedule;
+    return ok();
+}
+
 int lbm3d_step_bytes(const lbm3d* h, double* bytes) {
     if (h == nullptr || bytes == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null argument");
@@ -523,6 +586,10 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
             const bool local = (h->steps & 1U) != 0;
             rc = with_store(*h, [&]<class T>() -> int {
                 const auto p = descriptor<T>(*h);
+                if (!per_kernel) {
+                    const hipError_t err = step_once<T>(*h, p, local);
+                    return err == hipSuccess ? ok() : hip_fail("timed step", err);
+                }
                 hipError_t err = lbm::d3q19_step_bulk<T>(p, local, h->block, h->cpt, h->stream);
                 if (err == hipSuccess && per_kernel) {
                     err = hipEventRecord(ev[3U * n + 1U], h->stream);

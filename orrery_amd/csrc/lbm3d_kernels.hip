@@ -18,18 +18,142 @@ using d3::off3;
 // Bulk: every bulk node (all 18 neighbours inside the domain and fluid) streams plainly, CPT
 // cells per thread along x. Boundary: one thread per boundary node. kLocal selects the A step
 // (odd step count) or the B step (even).
-template <class T, bool kLocal, int CPT>
-__global__ __launch_bounds__(256) void d3q19_bulk(Lattice3<typename T::R> p) {
-    const std::uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+// V consecutive stored populations of one plane, as one memory access.
+template <class T, int V> __device__ __forceinline__ void load_run(const typename T::S* src, typename T::R (&out)[V]) {
+    typename T::S tmp[V];
+    __builtin_memcpy(tmp, src, sizeof(tmp));
 #pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const std::uint32_t cell = (tile * CPT + k) * blockDim.x + threadIdx.x;
-        if (cell >= p.cells) {
-            return;
+    for (int k = 0; k < V; ++k) {
+        out[k] = T::load(tmp[k]);
+    }
+}
+
+template <class T, int V> __device__ __forceinline__ void store_run(typename T::S* dst, const typename T::R (&in)[V]) {
+    typename T::S tmp[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        tmp[k] = T::pack(in[k]);
+    }
+    __builtin_memcpy(dst, tmp, sizeof(tmp));
+}
+
+// The bulk half of a step: each thread takes V consecutive cells and moves each population
+// plane as one V-wide access (19 loads and 19 stores per thread whatever V is). A run whose
+// cells are all bulk nodes stores whole runs; a mixed run stores its bulk cells one by one,
+// because the other cells' addresses belong to nodes the boundary kernel updates.
+template <class T, bool kLocal, int V>
+__global__ __launch_bounds__(256) void d3q19_bulk(Lattice3<typename T::R> p) {
+    using S = typename T::S;
+    using R = typename T::R;
+    const std::uint32_t c0 = (blockIdx.x * blockDim.x + threadIdx.x) * V;
+    if (c0 >= p.cells) {
+        return;
+    }
+    // The class array is padded with solid (0) entries up to the plane stride.
+    std::uint8_t kind[V];
+    __builtin_memcpy(kind, p.cls + c0, sizeof(kind));
+    bool any = false;
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < V; ++c) {
+        any = any || kind[c] == 1;
+        all = all && kind[c] == 1;
+    }
+    if (!any) {
+        return;
+    }
+    S* __restrict__ f = static_cast<S*>(p.f);
+    const std::uint32_t stride = p.stride;
+    const int nx = static_cast<int>(p.nx);
+    const int plane = static_cast<int>(p.nx * p.ny);
+    // B-step sources of a run reach nx ny + nx + 1 cells back (and V - 1 + that forward): the
+    // guard cells around the planes keep them in the allocation.
+    const std::uint32_t g = p.nx * p.ny + p.nx + 1U;
+    R v[V][19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        R run[V];
+        if constexpr (kLocal) {
+            load_run<T, V>(f + i * stride + c0, run);
+        } else {
+            load_run<T, V>(f - g + (L3::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(c0 + g) - d3::off3(i, nx, plane))), run);
         }
-        if (p.cls[cell] == 1) {
-            d3::bulk_node<T, kLocal>(p, cell);
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            v[c][i] = run[c];
         }
+    }
+#pragma unroll
+    for (int c = 0; c < V; ++c) {
+        if (kind[c] == 1) {
+            R sponge = 0;
+            if (p.sponge_strength > R(0)) {
+                const std::uint32_t cell = c0 + c;
+                const std::uint32_t z = p.div_plane.div(cell);
+                const std::uint32_t rest = cell - z * static_cast<std::uint32_t>(plane);
+                const std::uint32_t y = p.div_nx.div(rest);
+                sponge = d3::sponge_sigma3(p, rest - y * p.nx, y, z);
+            }
+            d3::collide3<T>(v[c], p, sponge);
+        }
+    }
+    if (all) {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            R run[V];
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                run[c] = v[c][i];
+            }
+            if constexpr (kLocal) {
+                store_run<T, V>(f + L3::opp(i) * stride + c0, run);
+            } else {
+                store_run<T, V>(f + (i * stride + static_cast<std::uint32_t>(static_cast<int>(c0) + d3::off3(i, nx, plane))), run);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            if (kind[c] == 1) {
+                const std::uint32_t cell = c0 + c;
+#pragma unroll
+                for (int i = 0; i < 19; ++i) {
+                    if constexpr (kLocal) {
+                        f[L3::opp(i) * stride + cell] = T::pack(v[c][i]);
+                    } else {
+                        f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + d3::off3(i, nx, plane))] = T::pack(v[c][i]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// The whole step in one launch, in cell order: each wave takes 64 consecutive cells, runs the
+// bulk update on its bulk lanes and then the boundary update on its boundary lanes. A boundary
+// node's slot is the number of boundary nodes before its 64-cell run (tile_slot) plus its rank
+// among the wave's boundary lanes. The boundary nodes of a row then share cache lines with the
+// bulk nodes around them, where a separate boundary pass would fetch a line per population for
+// each node of the inlet and outlet faces (which lie one row apart in memory).
+template <class T, bool kLocal>
+__global__ __launch_bounds__(256) void d3q19_fused(Lattice3<typename T::R> p) {
+    using R = typename T::R;
+    const std::uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool inside = cell < p.cells;
+    const std::uint32_t c = inside ? cell : p.cells - 1;
+    const std::uint8_t kind = inside ? p.cls[c] : std::uint8_t(0);
+    {
+        R v[19];
+        d3::bulk_load<T, kLocal>(p, c, v);
+        if (kind == 1) {
+            d3::bulk_finish<T, kLocal>(p, c, v);
+        }
+    }
+    const std::uint64_t mask = __ballot(kind == 2);
+    if (kind == 2) {
+        const std::uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<std::uint32_t>(mask >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(static_cast<std::uint32_t>(mask), 0U));
+        d3::boundary_node<T, kLocal>(p, p.tile_slot[c >> 6] + rank);
     }
 }
 
@@ -461,6 +585,16 @@ hipError_t d3q19_step_boundary(const Lattice3<typename T::R>& p, bool local, hip
     return hipGetLastError();
 }
 
+template <class T> hipError_t d3q19_step_fused(const Lattice3<typename T::R>& p, bool local, hipStream_t s) {
+    const std::uint32_t grid = blocks_for(p.cells, 256);
+    if (local) {
+        hipLaunchKernelGGL((d3q19_fused<T, true>), dim3(grid), dim3(256), 0, s, p);
+    } else {
+        hipLaunchKernelGGL((d3q19_fused<T, false>), dim3(grid), dim3(256), 0, s, p);
+    }
+    return hipGetLastError();
+}
+
 template <class T>
 hipError_t d3q19_step(const Lattice3<typename T::R>& p, bool local, int block, int cpt, hipStream_t s) {
     const hipError_t e = d3q19_step_bulk<T>(p, local, block, cpt, s);
@@ -532,6 +666,7 @@ hipError_t d3q19_force_sum(const Lattice3<typename T::R>& p, int phase, const st
     template hipError_t d3q19_step<T>(const Lattice3<T::R>&, bool, int, int, hipStream_t);                   \
     template hipError_t d3q19_step_bulk<T>(const Lattice3<T::R>&, bool, int, int, hipStream_t);              \
     template hipError_t d3q19_step_boundary<T>(const Lattice3<T::R>&, bool, hipStream_t);                    \
+    template hipError_t d3q19_step_fused<T>(const Lattice3<T::R>&, bool, hipStream_t);                       \
     template hipError_t d3q19_initialize<T>(const Lattice3<T::R>&, int, const double*, const double*,       \
                                             std::uint64_t, double, hipStream_t);                             \
     template hipError_t d3q19_populations<T>(const Lattice3<T::R>&, int, double*, hipStream_t);              \

@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lbm_device.h"
 
 namespace lbm::d3 {
@@ -55,62 +57,75 @@ __device__ __forceinline__ void collide3(typename T::R (&f)[19], const Lattice3<
                                          typename T::R sponge) {
     using R = typename T::R;
     if constexpr (T::shifted) {
-        // lbm_d3q19_common.glsl:264-326 and :358-368 (shifted equilibrium w (drho + rho (...)))
-        R drho = 0;
+        // The collision of lbm_d3q19_common.glsl:264-326 (shifted equilibrium
+        // w (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)), BGK, Smagorinsky, Guo forcing, sponge
+        // :358-368), evaluated for speed: opposite directions i, i + 1 (i odd) share
+        // cu^2, so their equilibria are sym +- asym; the non-equilibrium stress is
+        // P - rho u u - drho / 3 (the second moment of the equilibrium is exact on D3Q19);
+        // reciprocal and square root use the hardware instructions (1 ulp); fused
+        // multiply-adds are written out, so every kernel that inlines this evaluates it the
+        // same way. The results agree with the reference GPU arithmetic to binary32 rounding.
+        R ps[10];
+        R pd[10];
 #pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            drho += f[i];
+        for (int k = 1; k < 10; ++k) {
+            ps[k] = f[2 * k - 1] + f[2 * k];
+            pd[k] = f[2 * k - 1] - f[2 * k];
         }
-        const R jx = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14];
-        const R jy = f[3] - f[4] + f[7] - f[8] - f[9] + f[10] + f[15] - f[16] + f[17] - f[18];
-        const R jz = f[5] - f[6] + f[11] - f[12] - f[13] + f[14] + f[15] - f[16] - f[17] + f[18];
+        // pairs: 1 x, 2 y, 3 z, 4 (+x+y), 5 (+x-y), 6 (+x+z), 7 (+x-z), 8 (+y+z), 9 (+y-z)
+        const R drho = f[0] + (((ps[1] + ps[2]) + (ps[3] + ps[4])) + ((ps[5] + ps[6]) + (ps[7] + (ps[8] + ps[9]))));
+        const R jx = (pd[1] + pd[4]) + (pd[5] + (pd[6] + pd[7]));
+        const R jy = (pd[2] + pd[4]) - pd[5] + (pd[8] + pd[9]);
+        const R jz = (pd[3] + pd[6]) - pd[7] + (pd[8] - pd[9]);
         const R rho = R(1) + drho;
-        const R inv = R(1) / rho;
-        const R ux = (jx + R(0.5) * p.force[0]) * inv;
-        const R uy = (jy + R(0.5) * p.force[1]) * inv;
-        const R uz = (jz + R(0.5) * p.force[2]) * inv;
-        const R uu = ux * ux + uy * uy + uz * uz;
-        R feq[19];
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            const R cu = R(L3::cx(i)) * ux + R(L3::cy(i)) * uy + R(L3::cz(i)) * uz;
-            feq[i] = R(L3::w(i)) * (drho + rho * (R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu));
-        }
+        const R inv = __builtin_amdgcn_rcpf(rho);
+        const R ux = __builtin_fmaf(R(0.5), p.force[0], jx) * inv;
+        const R uy = __builtin_fmaf(R(0.5), p.force[1], jy) * inv;
+        const R uz = __builtin_fmaf(R(0.5), p.force[2], jz) * inv;
+        const R uu = __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, uz * uz));
         R omega = p.omega;
         if (p.les) {
-            R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+            const R third = drho * R(1.0 / 3.0);
+            const R rx = rho * ux;
+            const R ry = rho * uy;
+            const R rz = rho * uz;
+            const R qxx = __builtin_fmaf(-rx, ux, (ps[1] + ps[4]) + (ps[5] + (ps[6] + ps[7])) - third);
+            const R qyy = __builtin_fmaf(-ry, uy, (ps[2] + ps[4]) + (ps[5] + (ps[8] + ps[9])) - third);
+            const R qzz = __builtin_fmaf(-rz, uz, (ps[3] + ps[6]) + (ps[7] + (ps[8] + ps[9])) - third);
+            const R qxy = __builtin_fmaf(-rx, uy, ps[4] - ps[5]);
+            const R qxz = __builtin_fmaf(-rx, uz, ps[6] - ps[7]);
+            const R qyz = __builtin_fmaf(-ry, uz, ps[8] - ps[9]);
+            const R off = __builtin_fmaf(qxy, qxy, __builtin_fmaf(qxz, qxz, qyz * qyz));
+            const R strain = __builtin_amdgcn_sqrtf(
+                __builtin_fmaf(qxx, qxx, __builtin_fmaf(qyy, qyy, __builtin_fmaf(qzz, qzz, R(2) * off))));
+            // 18 sqrt(2) = 25.455844122715710
+            const R tau = R(0.5) * (p.tau + __builtin_amdgcn_sqrtf(__builtin_fmaf(
+                                                p.tau, p.tau, R(25.455844122715710) * p.smag_c2 * strain * inv)));
+            omega = __builtin_amdgcn_rcpf(tau);
+        }
+        const R base = __builtin_fmaf(R(-1.5) * rho, uu, drho);
+        f[0] = __builtin_fmaf(omega, R(1.0 / 3.0) * base - f[0], f[0]);
+        const R cu[10] = {R(0), ux, uy, uz, ux + uy, ux - uy, ux + uz, ux - uz, uy + uz, uy - uz};
+#pragma unroll
+        for (int k = 1; k < 10; ++k) {
+            const R w = k < 4 ? R(1.0 / 18.0) : R(1.0 / 36.0);
+            const R sym = __builtin_fmaf((R(4.5) * w) * rho * cu[k], cu[k], w * base);
+            const R asym = (R(3) * w) * rho * cu[k];
+            f[2 * k - 1] = __builtin_fmaf(omega, (sym + asym) - f[2 * k - 1], f[2 * k - 1]);
+            f[2 * k] = __builtin_fmaf(omega, (sym - asym) - f[2 * k], f[2 * k]);
+        }
+        if (p.forced) {
+            const R force_scale = R(1) - R(0.5) * omega;
 #pragma unroll
             for (int i = 0; i < 19; ++i) {
                 const R ex = R(L3::cx(i));
                 const R ey = R(L3::cy(i));
                 const R ez = R(L3::cz(i));
-                const R neq = f[i] - feq[i];
-                pxx += ex * ex * neq;
-                pyy += ey * ey * neq;
-                pzz += ez * ez * neq;
-                pxy += ex * ey * neq;
-                pxz += ex * ez * neq;
-                pyz += ey * ez * neq;
+                const R c = ex * ux + ey * uy + ez * uz;
+                f[i] += force_scale * R(L3::w(i)) *
+                        (R(3) * ((ex - ux) * p.force[0] + (ey - uy) * p.force[1] + (ez - uz) * p.force[2]) +
+                         R(9) * c * (ex * p.force[0] + ey * p.force[1] + ez * p.force[2]));
             }
-            const R strain =
-                sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
-            const R tau = R(0.5) * (p.tau + sqrt(p.tau * p.tau + R(25.455844122715710) * p.smag_c2 * strain / rho));
-            omega = R(1) / tau;
-        }
-        const R force_scale = R(1) - R(0.5) * omega;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            R value = f[i] - omega * (f[i] - feq[i]);
-            if (p.forced) {
-                const R ex = R(L3::cx(i));
-                const R ey = R(L3::cy(i));
-                const R ez = R(L3::cz(i));
-                const R cu = ex * ux + ey * uy + ez * uz;
-                value += force_scale * R(L3::w(i)) *
-                         (R(3) * ((ex - ux) * p.force[0] + (ey - uy) * p.force[1] + (ez - uz) * p.force[2]) +
-                          R(9) * cu * (ex * p.force[0] + ey * p.force[1] + ez * p.force[2]));
-            }
-            f[i] = value;
         }
         if (sponge > R(0)) {
             const R vx = p.inlet[0];
@@ -119,8 +134,8 @@ __device__ __forceinline__ void collide3(typename T::R (&f)[19], const Lattice3<
             const R vv = vx * vx + vy * vy + vz * vz;
 #pragma unroll
             for (int i = 0; i < 19; ++i) {
-                const R cu = R(L3::cx(i)) * vx + R(L3::cy(i)) * vy + R(L3::cz(i)) * vz;
-                const R target = R(L3::w(i)) * (R(3) * cu + R(4.5) * cu * cu - R(1.5) * vv);
+                const R c = R(L3::cx(i)) * vx + R(L3::cy(i)) * vy + R(L3::cz(i)) * vz;
+                const R target = R(L3::w(i)) * (R(3) * c + R(4.5) * c * c - R(1.5) * vv);
                 f[i] -= sponge * (f[i] - target);
             }
         }
@@ -272,41 +287,79 @@ __device__ __forceinline__ void inlet_west3(typename T::R (&f)[19], const Lattic
 // ---- one node of the step: bulk half ---------------------------------------------------------------------
 // Every bulk node (all 18 neighbours inside the domain and fluid) streams plainly: 19 loads,
 // the collision, 19 stores. kLocal selects the A step (odd n) or the B step (even n).
+//
+// The loads are split from the rest so that a kernel can issue them before it knows the
+// node's class (the class byte and the 19 populations then travel together instead of one
+// after the other). Any cell index below `cells` gives in-range addresses: the population
+// buffer has guard cells before plane 0 and after plane 18 (at least nx ny + nx + 1 each,
+// lbm_internal.h), so the B-step sources of a non-bulk node, whose values are discarded,
+// stay inside the allocation. Indices are formed from a base moved back by g = nx ny + nx + 1
+// so that they stay unsigned.
 template <class T, bool kLocal>
-__device__ __forceinline__ void bulk_node(const Lattice3<typename T::R>& p, std::uint32_t cell) {
+__device__ __forceinline__ void bulk_load(const Lattice3<typename T::R>& p, std::uint32_t cell,
+                                          typename T::R (&v)[19]) {
+    using S = typename T::S;
+    const S* __restrict__ f = static_cast<const S*>(p.f);
+    const std::uint32_t stride = p.stride;
+    const int nx = static_cast<int>(p.nx);
+    const int plane = static_cast<int>(p.nx * p.ny);
+    if constexpr (kLocal) {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            v[i] = T::load(f[i * stride + cell]);
+        }
+    } else {
+        const std::uint32_t g = p.nx * p.ny + p.nx + 1U;
+        const S* __restrict__ base = f - g;
+        const std::uint32_t shifted = cell + g;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            v[i] = T::load(base[L3::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(shifted) - off3(i, nx, plane))]);
+        }
+    }
+}
+
+template <class T, bool kLocal>
+__device__ __forceinline__ void bulk_finish(const Lattice3<typename T::R>& p, std::uint32_t cell,
+                                            typename T::R (&v)[19]) {
     using S = typename T::S;
     using R = typename T::R;
     S* __restrict__ f = static_cast<S*>(p.f);
     const std::uint32_t stride = p.stride;
     const int nx = static_cast<int>(p.nx);
     const int plane = static_cast<int>(p.nx * p.ny);
-    {
-        R v[19];
+    R sponge = 0;
+    if (p.sponge_strength > R(0)) {
+        const std::uint32_t z = p.div_plane.div(cell);
+        const std::uint32_t rest = cell - z * static_cast<std::uint32_t>(plane);
+        const std::uint32_t y = p.div_nx.div(rest);
+        sponge = sponge_sigma3(p, rest - y * p.nx, y, z);
+    }
+    collide3<T>(v, p, sponge);
 #pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            if constexpr (kLocal) {
-                v[i] = T::load(f[i * stride + cell]);
-            } else {
-                v[i] = T::load(f[L3::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(cell) - off3(i, nx, plane))]);
-            }
-        }
-        R sponge = 0;
-        if (p.sponge_strength > R(0)) {
-            const std::uint32_t z = p.div_plane.div(cell);
-            const std::uint32_t rest = cell - z * static_cast<std::uint32_t>(plane);
-            const std::uint32_t y = p.div_nx.div(rest);
-            sponge = sponge_sigma3(p, rest - y * p.nx, y, z);
-        }
-        collide3<T>(v, p, sponge);
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            if constexpr (kLocal) {
-                f[L3::opp(i) * stride + cell] = T::pack(v[i]);
-            } else {
-                f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] = T::pack(v[i]);
-            }
+    for (int i = 0; i < 19; ++i) {
+        if constexpr (kLocal) {
+            f[L3::opp(i) * stride + cell] = T::pack(v[i]);
+        } else {
+            f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] = T::pack(v[i]);
         }
     }
+}
+
+template <class T, bool kLocal>
+__device__ __forceinline__ void bulk_node(const Lattice3<typename T::R>& p, std::uint32_t cell) {
+    typename T::R v[19];
+    bulk_load<T, kLocal>(p, cell, v);
+    bulk_finish<T, kLocal>(p, cell, v);
+}
+
+// Direction i after the mirrors a source code asks for, chosen among compile-time constants
+// (i is a constant of an unrolled loop) rather than read from a table at run time.
+__device__ __forceinline__ std::uint32_t mirrored(int i, std::uint32_t code) {
+    const bool my = (code & kCodeMirrorY) != 0U;
+    const bool mz = (code & kCodeMirrorZ) != 0U;
+    return static_cast<std::uint32_t>(my ? (mz ? L3::mirror_z(L3::mirror_y(i)) : L3::mirror_y(i))
+                                         : (mz ? L3::mirror_z(i) : i));
 }
 
 // ---- one node of the step: boundary half -----------------------------------------------------------------
@@ -349,10 +402,7 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
             if ((code & (kCodeMirrorY | kCodeMirrorZ)) == 0U) {
                 v[i] = natural(i, payload);
             } else {
-                int d = i;
-                if (code & kCodeMirrorY) d = L3::mirror_y(d);
-                if (code & kCodeMirrorZ) d = L3::mirror_z(d);
-                v[i] = st[d * bs + payload];
+                v[i] = st[mirrored(i, code) * bs + payload];
             }
         } else if (kind == kBounce) {
             // reflected (lbm3d_internal.h:185-197)
@@ -376,14 +426,12 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
                 }
                 wall_rho = T::shifted ? R(1) + s : s;
             }
-            v[i] = st[L3::opp(i) * bs + b] + R(6) * R(L3::w(i)) * wall_rho * R(L3::cx(i)) * p.wall_u[payload];
+            const R u = payload == 0U ? p.wall_u[0] : (payload == 1U ? p.wall_u[1] : (payload == 2U ? p.wall_u[2] : p.wall_u[3]));
+            v[i] = st[L3::opp(i) * bs + b] + R(6) * R(L3::w(i)) * wall_rho * R(L3::cx(i)) * u;
         } else if (kind == kInlet) {
             v[i] = R(0);
         } else {
-            int d = i;
-            if (code & kCodeMirrorY) d = L3::mirror_y(d);
-            if (code & kCodeMirrorZ) d = L3::mirror_z(d);
-            R up = st[d * bs + payload];
+            R up = st[mirrored(i, code) * bs + payload];
             if (p.convective) {
                 R* h = p.hist + L3::outlet_slot(i) * bs + b;
                 up = (*h + p.outlet_speed * up) / (R(1) + p.outlet_speed);
@@ -409,14 +457,14 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
             int tz = static_cast<int>(z) + L3::cz(i);
             bool inside = true;
             if (ty < 0 || ty >= static_cast<int>(p.ny)) {
-                if (p.side[ty < 0 ? 0 : 1] == kPeriodic) {
+                if ((ty < 0 ? p.side[0] : p.side[1]) == kPeriodic) {
                     ty = ty < 0 ? ty + static_cast<int>(p.ny) : ty - static_cast<int>(p.ny);
                 } else {
                     inside = false;
                 }
             }
             if (tz < 0 || tz >= static_cast<int>(p.nz)) {
-                if (p.side[tz < 0 ? 2 : 3] == kPeriodic) {
+                if ((tz < 0 ? p.side[2] : p.side[3]) == kPeriodic) {
                     tz = tz < 0 ? tz + static_cast<int>(p.nz) : tz - static_cast<int>(p.nz);
                 } else {
                     inside = false;

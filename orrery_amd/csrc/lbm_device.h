@@ -104,6 +104,7 @@ template <typename R> struct Lattice3 {
     const R* bq;                 // [19][bstride] wall fractions (interpolated wall) or null
     R* stash;                    // [2][19][bstride]
     R* hist;                     // [5][bstride] convective-outlet history (outlet nodes)
+    const std::uint32_t* tile_slot;  // [cells / 64 + 1] boundary nodes before each 64-cell run
     std::uint32_t nx, ny, nz, cells, stride, nb, bstride;
     FastDiv div_nx, div_plane;
     int x_mode;       // 0 periodic, 1 inflow/outflow

@@ -42,6 +42,9 @@ struct Tables {
     std::vector<double> bq;              // [Q][bstride] wall fractions (interpolated wall only)
     std::vector<Link> links;             // boundary_links order
     std::vector<std::uint32_t> lslot;    // boundary slot of each link's node
+    // D3Q19: boundary nodes before each run of 64 cells ([cells / 64 + 1]); the fused step
+    // kernel finds a boundary node's slot as tile_slot[cell / 64] + its rank in the wave.
+    std::vector<std::uint32_t> tile_slot;
     std::uint32_t bstride = 0;
 };
 

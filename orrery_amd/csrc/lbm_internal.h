@@ -22,6 +22,15 @@ struct lbm3d {
     lbm::host::Tables tables;
     std::vector<std::uint8_t> solid;
     hipStream_t stream = nullptr;
+    // How a step is launched (LBM_SCHEDULE_*). With the two-stream schedule the boundary
+    // kernel runs on `side`, concurrently with the bulk kernel on `stream`: within one step
+    // every population address is read and written by one node only, and the stash is
+    // double-buffered, so the two kernels are independent. `fork` and `join` order each step
+    // after the previous one on both streams.
+    int schedule = LBM_SCHEDULE_SERIAL;
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr;
+    hipEvent_t join = nullptr;
     int device = 0;
     std::uint64_t steps = 0;
     double inlet[3] = {0, 0, 0};
@@ -39,6 +48,7 @@ struct lbm3d {
     std::uint8_t* ldir = nullptr;
     double* lq = nullptr;
     std::uint32_t* lslot = nullptr;
+    std::uint32_t* tile_slot = nullptr;
     double* partial = nullptr;
     double* result = nullptr;
     std::size_t device_bytes = 0;
@@ -61,15 +71,21 @@ inline std::size_t store_bytes(int storage) {
 
 inline std::size_t real_bytes(int storage) { return storage == LBM_STORE_F64 ? 8 : 4; }
 
+// Guard cells before and after the population planes: the B step of the bulk kernel loads the
+// sources of every cell of its tile before it knows whether the cell is a bulk node, and those
+// sources lie at most nx ny + nx + 1 cells away. Rounded to 128 cells to keep the planes aligned.
+inline std::size_t guard_cells(std::size_t nx, std::size_t ny) { return lbm::host::padded(nx * ny + nx + 1, 128); }
+
 // Device bytes of a lattice with these tables (the admission rule of the batch engine and of
 // the memory check; the counterpart of gpu_memory_bytes, lbm3d_common.cpp:276-300).
-inline std::size_t lattice_bytes(std::size_t cells, std::uint32_t stride, const lbm::host::Tables& t, int storage,
-                                 bool interpolated) {
+inline std::size_t lattice_bytes(std::size_t cells, std::uint32_t stride, std::size_t guard, const lbm::host::Tables& t,
+                                 int storage, bool interpolated) {
     const std::size_t sb = store_bytes(storage);
     const std::size_t rb = real_bytes(storage);
-    std::size_t total = 19U * static_cast<std::size_t>(stride) * sb; // populations (in place)
-    total += cells;                                                     // node classes (u8)
+    std::size_t total = (19U * static_cast<std::size_t>(stride) + 2U * guard) * sb; // populations (in place)
+    total += stride;                                                    // node classes (u8, padded)
     total += t.bcell.size() * 4U;                                       // boundary list
+    total += t.tile_slot.size() * 4U;                                   // boundary slot per 64 cells
     total += 19U * static_cast<std::size_t>(t.bstride) * 4U;            // source codes
     if (interpolated) {
         total += 19U * static_cast<std::size_t>(t.bstride) * rb; // wall fractions
@@ -88,7 +104,8 @@ inline std::size_t required_bytes3(const lbm3d_config& c, const std::uint8_t* so
                                    int storage) {
     const std::size_t cells = c.nx * c.ny * c.nz;
     const lbm::host::Tables t = lbm::host::tables3(c, solid, distance);
-    return lattice_bytes(cells, lbm::host::padded(cells, 128), t, storage, c.solid_wall == LBM_SOLID_INTERPOLATED);
+    return lattice_bytes(cells, lbm::host::padded(cells, 128), guard_cells(c.nx, c.ny), t, storage,
+                         c.solid_wall == LBM_SOLID_INTERPOLATED);
 }
 
 template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
@@ -102,6 +119,7 @@ template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
     p.bq = static_cast<const R*>(h.bq);
     p.stash = static_cast<R*>(h.stash);
     p.hist = static_cast<R*>(h.hist);
+    p.tile_slot = h.tile_slot;
     p.nx = static_cast<std::uint32_t>(c.nx);
     p.ny = static_cast<std::uint32_t>(c.ny);
     p.nz = static_cast<std::uint32_t>(c.nz);

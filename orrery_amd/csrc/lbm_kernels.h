@@ -19,6 +19,9 @@ template <class T>
 hipError_t d3q19_step_bulk(const Lattice3<typename T::R>& p, bool local, int block, int cpt, hipStream_t s);
 template <class T>
 hipError_t d3q19_step_boundary(const Lattice3<typename T::R>& p, bool local, hipStream_t s);
+// The whole step in one launch (bulk and boundary nodes in cell order).
+template <class T>
+hipError_t d3q19_step_fused(const Lattice3<typename T::R>& p, bool local, hipStream_t s);
 // mode 0 uniform u0, 1 moment planes (device), 2 u0 + seeded perturbation.
 template <class T>
 hipError_t d3q19_initialize(const Lattice3<typename T::R>& p, int mode, const double* planes, const double u0[3],

@@ -344,11 +344,18 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
     t.bcell.reserve(nb);
     std::vector<std::uint32_t> slot_of;
     slot_of.assign(cells, 0xffffffffU);
+    t.tile_slot.assign(cells / 64 + 1, 0U);
     for (std::size_t cell = 0; cell < cells; ++cell) {
+        if ((cell & 63U) == 0) {
+            t.tile_slot[cell >> 6] = static_cast<std::uint32_t>(t.bcell.size());
+        }
         if (t.cls[cell] == 2) {
             slot_of[cell] = static_cast<std::uint32_t>(t.bcell.size());
             t.bcell.push_back(static_cast<std::uint32_t>(cell));
         }
+    }
+    if ((cells & 63U) == 0) {
+        t.tile_slot[cells >> 6] = static_cast<std::uint32_t>(t.bcell.size());
     }
     t.bstride = padded(nb, 64);
     t.bcode.assign(static_cast<std::size_t>(19) * t.bstride, 0U);

@@ -219,6 +219,11 @@ class Lbm3DSolver:
     def set_launch(self, block: int = 0, cells_per_thread: int = 0) -> None:
         _check(self._lib, self._lib.lbm3d_set_launch(self._h, block, cells_per_thread))
 
+    def set_schedule(self, schedule: str) -> None:
+        """"fused" (one kernel per step, the default), "serial" (bulk then boundary kernel) or
+        "streams" (the two kernels concurrently on two streams); all give identical results."""
+        _check(self._lib, self._lib.lbm3d_set_schedule(self._h, abi.SCHEDULE[schedule]))
+
     def step_bytes(self) -> float:
         v = ctypes.c_double()
         _check(self._lib, self._lib.lbm3d_step_bytes(self._h, ctypes.byref(v)))

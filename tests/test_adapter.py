@@ -1,0 +1,63 @@
+"""The C++ drop-in (include/orrery_lbm_solver.hpp): HipLbm3D implements the reference's
+Lbm3DSolver interface over the C ABI. Compiled against the reference's own headers where the
+reference tree is present; its refusals carry the reference's validate() messages, compared
+with the reference's validate compiled from its sources (oracle/_ref)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+import refcheck
+from orrery_amd import Lbm3DConfig, Wall, WallKind, SolidWall, fluid_mask_3d
+from orrery_amd.abi import LIBRARY
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+REFERENCE = "/root/reference"
+INCLUDES = ["src/aero/lbm3d/include", "src/aero/lbm/include", "src/aero/compute/include", "src/core/include",
+            "src/geom/include"]
+
+pytestmark = pytest.mark.skipif(not (os.path.isdir(REFERENCE) and shutil.which("g++") and refcheck.available()),
+                                reason="needs the reference's headers and oracle/_ref")
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("adapter") / "adapter_check")
+    cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include")]
+    for inc in INCLUDES:
+        cmd += ["-I", os.path.join(REFERENCE, inc)]
+    cmd += [os.path.join(HERE, "adapter_check.cpp"), "-o", exe, LIBRARY, f"-Wl,-rpath,{os.path.dirname(LIBRARY)}"]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return exe
+
+
+def config(what):
+    c = Lbm3DConfig(nx=16, ny=8, nz=8, tau=0.6, inlet_ux=0.05)
+    m = fluid_mask_3d(16, 8, 8)
+    if what == "tau":
+        c.tau = 0.5
+    elif what == "mach":
+        c.inlet_ux = 0.2
+    elif what == "small":
+        c.nz = 2
+        m = fluid_mask_3d(16, 8, 2)
+    elif what == "distance":
+        c.solid_wall = SolidWall.interpolated
+    elif what == "periodic":
+        c.y_low = Wall(WallKind.periodic)
+    return c, m
+
+
+@pytest.mark.parametrize("what", ["tau", "mach", "small", "distance", "periodic"])
+def test_refusals_are_the_references(driver, what):
+    out = subprocess.run([driver, what], check=True, capture_output=True, text=True).stdout.strip()
+    name, message = out.split("\t")
+    assert name == what
+    assert message == refcheck.validate3(*config(what))
+
+
+def test_mask_shape_is_checked_before_the_pointer_crosses(driver):
+    out = subprocess.run([driver, "size"], check=True, capture_output=True, text=True).stdout.strip()
+    assert out == "size\tlbm3d: the solid mask does not match the lattice size"
