@@ -128,6 +128,12 @@ int lbm3d_mass(lbm3d* h, double* mass);
 /* Lbm3DSolver::macroscopic (lbm3d_cpu.cpp:154-186): density and velocity (Guo half-force
  * corrected) per node; solids report 1 and 0. Any pointer may be NULL. */
 int lbm3d_macroscopic(lbm3d* h, double* rho, double* ux, double* uy, double* uz);
+/* macroscopic() at n listed nodes only, computed on the device: 4 n doubles cross the bus
+ * instead of 4 cells (the adapter's wall-pressure samples and inlet-plane density). Any output
+ * may be NULL; values are those lbm3d_macroscopic reports at the same nodes. */
+int lbm3d_sample(lbm3d* h, const uint32_t* cells, size_t n, double* rho, double* ux, double* uy, double* uz);
+/* max_speed(macroscopic()) (lbm3d_common.cpp:200-211), reduced on the device. */
+int lbm3d_max_speed(lbm3d* h, double* speed);
 /* Lbm3DSolver::populations (lbm3d_gpu.cpp:394-405): 19 * cells doubles, logical layout. */
 int lbm3d_populations(lbm3d* h, double* f);
 /* Lbm3DSolver::links (boundary_links, lbm3d_common.cpp:176-198): writes up to `capacity`
@@ -288,6 +294,96 @@ int lbm_signed_distance_band(const lbm_mesh* m, const double origin[3], double s
                              uint8_t* in_band);
 /* MeshTree::winding_number at n points [n][3]. */
 int lbm_winding_numbers(const lbm_mesh* m, const double* points, size_t n, double* out);
+
+/* ---- the aerodynamic adapter (LbmAeroSolver3D, src/aero/lbm3d/lbm_aero_solver3d.h:45-166) ---- */
+/* A closed body mesh in SI units (body axes, metres) goes in; coefficients, SI force and moment,
+ * the force history, wall-pressure samples and validity warnings come out. The lattice is
+ * wind-aligned (x along the free stream, y the side force, z the lift); the body is voxelised
+ * with the geometry above and placed `upstream` reference lengths behind a velocity inlet and
+ * `downstream` ahead of a convective outlet, in a square channel `width` lengths wide.
+ * Errors: lbm_aero3d_last_error(). */
+enum { LBM_LES_AUTOMATIC = 0, LBM_LES_ON = 1, LBM_LES_OFF = 2 }; /* LesMode, lbm_aero_solver.h:32-36 */
+
+typedef struct lbm_aero3d_options { /* LbmSolver3DOptions, lbm_aero_solver3d.h:45-81 */
+    double cells_per_length, lattice_velocity; /* LatticeResolution{16, 0.05} */
+    double upstream, downstream, width;        /* reference lengths: 5, 10, 6 */
+    int32_t sides;                             /* LBM_WALL_SLIP or LBM_WALL_PERIODIC */
+    int32_t wall;                              /* LBM_SOLID_INTERPOLATED (default) or _HALFWAY */
+    int32_t les;                               /* LBM_LES_* */
+    double smagorinsky_constant;               /* 0.1 */
+    double duration, average_window, start_ramp; /* convective times: 30, 8, 3 */
+    double sponge_strength, sponge_inlet, sponge_outlet, sponge_sides; /* 0.1; 0, 2, 0 lengths */
+    uint64_t steps_per_call, sample_every;     /* 200, 10 */
+    uint64_t max_cells;                        /* 0: the device memory, else 2^24 */
+    int32_t storage;                           /* LBM_STORE_* of the lattice (no reference counterpart) */
+} lbm_aero3d_options;
+
+typedef struct lbm_flow_conditions { /* FlowConditions, src/aero/include/orrery/aero/flow_conditions.h */
+    double speed, density, viscosity, temperature, angle_of_attack, sideslip, gamma, gas_constant;
+} lbm_flow_conditions;
+
+typedef struct lbm_reference_values { /* ReferenceValues, flow_conditions.h */
+    double area, length, moment_centre[3];
+} lbm_reference_values;
+
+typedef struct lbm_aero3d_result { /* the scalars of AeroResult, src/aero/include/orrery/aero/aero_result.h */
+    double cd, cl, cy, cm;
+    double force[3], moment[3]; /* N and N m, body axes, moment about the moment centre */
+    uint64_t iterations;
+    int32_t converged;
+    double residual;            /* last relative change of cd */
+    size_t samples, wall_samples, warnings;
+} lbm_aero3d_result;
+
+typedef struct lbm_aero3d_info { /* what setup() decided */
+    lbm3d_config config;
+    double origin[3];  /* wind-frame position (m) of lattice node (0, 0, 0) */
+    double centre[3];  /* moment centre, lattice coordinates */
+    uint64_t first[3], count[3]; /* voxelised box inside the lattice */
+    double axes[9];    /* wind axes (drag, side, lift) in body coordinates, row by row */
+    double dx, dt, reynolds, lattice_velocity, cells_per_length;
+    uint64_t total_steps;
+    size_t window_samples;
+    double ramp_steps, blockage, surface_area;
+    size_t solid_nodes;
+} lbm_aero3d_info;
+
+typedef struct lbm_aero3d lbm_aero3d;
+const char* lbm_aero3d_last_error(void);
+void lbm_aero3d_default_options(lbm_aero3d_options* options);
+void lbm_default_flow_conditions(lbm_flow_conditions* conditions);
+int lbm_aero3d_create(const lbm_aero3d_options* options, lbm_aero3d** out);
+int lbm_aero3d_destroy(lbm_aero3d* a);
+/* setup (lbm_aero_solver3d.cpp:85-339) without the device lattice: the lattice configuration,
+ * mask and warnings only, on the host. */
+int lbm_aero3d_plan(lbm_aero3d* a, const lbm_mesh* mesh, const lbm_reference_values* reference,
+                    const lbm_flow_conditions* conditions);
+/* setup: plan, then the device lattice (lbm3d_create) and the gradual start. */
+int lbm_aero3d_setup(lbm_aero3d* a, const lbm_mesh* mesh, const lbm_reference_values* reference,
+                     const lbm_flow_conditions* conditions);
+/* step (lbm_aero_solver3d.cpp:342-411): *finished = 1 once the run is complete. */
+int lbm_aero3d_step(lbm_aero3d* a, int32_t* finished);
+int lbm_aero3d_get_result(const lbm_aero3d* a, lbm_aero3d_result* out);
+/* Force history: sample time (lattice steps), drag, lift and side-force coefficients. */
+int lbm_aero3d_history(const lbm_aero3d* a, double* time, double* drag, double* lift, double* side, size_t capacity,
+                       size_t* count);
+int lbm_aero3d_residuals(const lbm_aero3d* a, double* residuals, size_t capacity, size_t* count);
+/* Coefficients (drag, lift, side) of the last sample. */
+int lbm_aero3d_instantaneous(const lbm_aero3d* a, double coefficients[3]);
+/* Wall samples (SurfaceField): points and normals [n][3], body axes; areas; cp. */
+int lbm_aero3d_surface(const lbm_aero3d* a, double* points, double* normals, double* areas, double* cp,
+                       size_t capacity, size_t* count);
+int lbm_aero3d_warning_count(const lbm_aero3d* a, size_t* count);
+int lbm_aero3d_warning(const lbm_aero3d* a, size_t index, char* code, size_t code_capacity, char* message,
+                       size_t message_capacity);
+int lbm_aero3d_get_info(const lbm_aero3d* a, lbm_aero3d_info* out);
+/* The lattice mask of the plan: solid [cells]; distance [cells] for the interpolated wall. */
+int lbm_aero3d_mask(const lbm_aero3d* a, uint8_t* solid, double* distance);
+/* The device lattice (borrowed; valid until the next setup or destroy). */
+int lbm_aero3d_lattice(const lbm_aero3d* a, lbm3d** out);
+/* The SI velocity field [cells][3] on the lattice, on request (zero angle of attack and
+ * sideslip only). */
+int lbm_aero3d_velocity(lbm_aero3d* a, double* values);
 
 /* ---- shared helpers ----------------------------------------------------------------------- */
 /* Seeded perturbation in [-1, 1): SplitMix64 of (seed, cell, component), 53-bit mantissa.

@@ -56,6 +56,42 @@ class COscillation(ctypes.Structure):
                 ("amplitude", ctypes.c_double), ("period_spread", ctypes.c_double)]
 
 
+D = ctypes.c_double
+U64 = ctypes.c_uint64
+I32 = ctypes.c_int32
+
+
+class CAeroOptions(ctypes.Structure):
+    """lbm_aero3d_options (LbmSolver3DOptions, lbm_aero_solver3d.h:45-81)."""
+    _fields_ = [("cells_per_length", D), ("lattice_velocity", D), ("upstream", D), ("downstream", D), ("width", D),
+                ("sides", I32), ("wall", I32), ("les", I32), ("smagorinsky_constant", D), ("duration", D),
+                ("average_window", D), ("start_ramp", D), ("sponge_strength", D), ("sponge_inlet", D),
+                ("sponge_outlet", D), ("sponge_sides", D), ("steps_per_call", U64), ("sample_every", U64),
+                ("max_cells", U64), ("storage", I32)]
+
+
+class CFlowConditions(ctypes.Structure):
+    _fields_ = [(n, D) for n in ("speed", "density", "viscosity", "temperature", "angle_of_attack", "sideslip",
+                                 "gamma", "gas_constant")]
+
+
+class CReferenceValues(ctypes.Structure):
+    _fields_ = [("area", D), ("length", D), ("moment_centre", D * 3)]
+
+
+class CAeroResult(ctypes.Structure):
+    _fields_ = [("cd", D), ("cl", D), ("cy", D), ("cm", D), ("force", D * 3), ("moment", D * 3),
+                ("iterations", U64), ("converged", I32), ("residual", D), ("samples", ctypes.c_size_t),
+                ("wall_samples", ctypes.c_size_t), ("warnings", ctypes.c_size_t)]
+
+
+class CAeroInfo(ctypes.Structure):
+    _fields_ = [("config", CConfig3D), ("origin", D * 3), ("centre", D * 3), ("first", U64 * 3), ("count", U64 * 3),
+                ("axes", D * 9), ("dx", D), ("dt", D), ("reynolds", D), ("lattice_velocity", D),
+                ("cells_per_length", D), ("total_steps", U64), ("window_samples", ctypes.c_size_t),
+                ("ramp_steps", D), ("blockage", D), ("surface_area", D), ("solid_nodes", ctypes.c_size_t)]
+
+
 P = ctypes.POINTER
 c_u8p = P(ctypes.c_uint8)
 c_u32p = P(ctypes.c_uint32)
@@ -79,6 +115,9 @@ EXPORTS = [
     ("lbm3d_force", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p]),
     ("lbm3d_mass", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_macroscopic", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p, c_f64p]),
+    ("lbm3d_sample", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint32), ctypes.c_size_t, c_f64p, c_f64p, c_f64p,
+                                    c_f64p]),
+    ("lbm3d_max_speed", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_populations", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_links", ctypes.c_int, [ctypes.c_void_p, c_u32p, c_u32p, c_f64p, ctypes.c_size_t, c_sizep]),
     ("lbm3d_device_bytes", ctypes.c_int, [ctypes.c_void_p, c_sizep]),
@@ -150,6 +189,28 @@ EXPORTS = [
                                                 ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint8),
                                                 ctypes.c_double, c_f64p, P(ctypes.c_uint8)]),
     ("lbm_winding_numbers", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_size_t, c_f64p]),
+    ("lbm_aero3d_last_error", ctypes.c_char_p, []),
+    ("lbm_aero3d_default_options", None, [P(CAeroOptions)]),
+    ("lbm_default_flow_conditions", None, [P(CFlowConditions)]),
+    ("lbm_aero3d_create", ctypes.c_int, [P(CAeroOptions), P(ctypes.c_void_p)]),
+    ("lbm_aero3d_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm_aero3d_plan", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(CReferenceValues), P(CFlowConditions)]),
+    ("lbm_aero3d_setup", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(CReferenceValues), P(CFlowConditions)]),
+    ("lbm_aero3d_step", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int32)]),
+    ("lbm_aero3d_get_result", ctypes.c_int, [ctypes.c_void_p, P(CAeroResult)]),
+    ("lbm_aero3d_history", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p, c_f64p, ctypes.c_size_t,
+                                          P(ctypes.c_size_t)]),
+    ("lbm_aero3d_residuals", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_size_t, P(ctypes.c_size_t)]),
+    ("lbm_aero3d_instantaneous", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm_aero3d_surface", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p, c_f64p, ctypes.c_size_t,
+                                          P(ctypes.c_size_t)]),
+    ("lbm_aero3d_warning_count", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_size_t)]),
+    ("lbm_aero3d_warning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t,
+                                          ctypes.c_char_p, ctypes.c_size_t]),
+    ("lbm_aero3d_get_info", ctypes.c_int, [ctypes.c_void_p, P(CAeroInfo)]),
+    ("lbm_aero3d_mask", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint8), c_f64p]),
+    ("lbm_aero3d_lattice", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_void_p)]),
+    ("lbm_aero3d_velocity", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
 ]
 
 _lib = None

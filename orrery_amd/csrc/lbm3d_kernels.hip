@@ -15,9 +15,9 @@ using L3 = D3Q19;
 using d3::off3;
 
 // ---- the step kernels -------------------------------------------------------------------------
-// Bulk: every bulk node (all 18 neighbours inside the domain and fluid) streams plainly, CPT
-// cells per thread along x. Boundary: one thread per boundary node. kLocal selects the A step
-// (odd step count) or the B step (even).
+// Bulk: every bulk node (all 18 neighbours inside the domain and fluid) streams plainly, V
+// consecutive cells per thread. Boundary: one thread per boundary node. kLocal selects the A
+// step (odd step count) or the B step (even).
 // V consecutive stored populations of one plane, as one memory access.
 template <class T, int V> __device__ __forceinline__ void load_run(const typename T::S* src, typename T::R (&out)[V]) {
     typename T::S tmp[V];
@@ -500,6 +500,125 @@ __global__ __launch_bounds__(256) void d3q19_mass_boundary(Lattice3<typename T::
     block_sum<1>(v, partial);
 }
 
+// Moments of listed nodes (the values macroscopic() reports there): a node with a boundary
+// slot reads its stash entries, a bulk node its lattice entries, anything else is at rest.
+template <class T>
+__global__ __launch_bounds__(256) void d3q19_sample(Lattice3<typename T::R> p, int phase, double fx, double fy, double fz,
+                                                    const std::uint32_t* cells, const std::int32_t* slots, std::uint32_t n,
+                                                    double* out) {
+    const std::uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) {
+        return;
+    }
+    const std::uint32_t cell = cells[k];
+    const std::int32_t slot = slots[k];
+    const double force[3] = {fx, fy, fz};
+    double rho = 1.0, ux = 0.0, uy = 0.0, uz = 0.0;
+    double v[19];
+    if (slot >= 0) {
+        const auto* st = p.stash + (phase ? 19U * p.bstride : 0U);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            v[i] = static_cast<double>(st[i * p.bstride + static_cast<std::uint32_t>(slot)]);
+        }
+        moments_of<T>(v, force, rho, ux, uy, uz);
+    } else if (cell < p.cells && p.cls[cell] == 1) {
+        const auto* f = static_cast<const typename T::S*>(p.f);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            v[i] = static_cast<double>(bulk_value<T>(p, f, i, cell, phase));
+        }
+        moments_of<T>(v, force, rho, ux, uy, uz);
+    }
+    out[k] = rho;
+    out[n + k] = ux;
+    out[2U * n + k] = uy;
+    out[3U * n + k] = uz;
+}
+
+// Largest |u| of one block (exact: a maximum does not depend on the order it is taken in).
+__device__ __forceinline__ void block_max(double v, double* partial) {
+    __shared__ double lds[16];
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        v = fmax(v, __shfl_down(v, offset, 64));
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        const int waves = static_cast<int>(blockDim.x >> 6);
+        for (int w = 0; w < waves; ++w) {
+            m = fmax(m, lds[w]);
+        }
+        partial[blockIdx.x] = m;
+    }
+}
+
+// max_speed (lbm3d_common.cpp:200-211): sqrt(u^2 + v^2 + w^2) of the moments of each fluid node.
+template <class T>
+__global__ __launch_bounds__(256) void d3q19_speed_cells(Lattice3<typename T::R> p, int phase, double fx, double fy,
+                                                         double fz, double* partial) {
+    const std::uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    double speed = 0.0;
+    if (cell < p.cells && p.cls[cell] == 1) {
+        const auto* f = static_cast<const typename T::S*>(p.f);
+        double v[19];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            v[i] = static_cast<double>(bulk_value<T>(p, f, i, cell, phase));
+        }
+        const double force[3] = {fx, fy, fz};
+        double rho, ux, uy, uz;
+        moments_of<T>(v, force, rho, ux, uy, uz);
+        speed = sqrt(ux * ux + uy * uy + uz * uz);
+    }
+    block_max(speed, partial);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void d3q19_speed_boundary(Lattice3<typename T::R> p, int phase, double fx, double fy,
+                                                            double fz, double* partial) {
+    const std::uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    double speed = 0.0;
+    if (b < p.nb) {
+        const auto* st = p.stash + (phase ? 19U * p.bstride : 0U);
+        double v[19];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            v[i] = static_cast<double>(st[i * p.bstride + b]);
+        }
+        const double force[3] = {fx, fy, fz};
+        double rho, ux, uy, uz;
+        moments_of<T>(v, force, rho, ux, uy, uz);
+        speed = sqrt(ux * ux + uy * uy + uz * uz);
+    }
+    block_max(speed, partial);
+}
+
+__global__ __launch_bounds__(256) void reduce_max(const double* partial, std::uint32_t n, double* out) {
+    __shared__ double lds[256];
+    double m = 0.0;
+    for (std::uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        m = fmax(m, partial[k]);
+    }
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out = lds[0];
+    }
+}
+
 } // namespace
 
 // Final stage of the FP64 reductions: one block sums `n` partials of K values in a fixed
@@ -639,6 +758,34 @@ hipError_t d3q19_moments(const Lattice3<typename T::R>& p, int phase, const doub
     return hipGetLastError();
 }
 
+template <class T>
+hipError_t d3q19_sample(const Lattice3<typename T::R>& p, int phase, const double force[3], const std::uint32_t* cells,
+                        const std::int32_t* slots, std::uint32_t n, double* out, hipStream_t s) {
+    if (n == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL((d3q19_sample<T>), dim3(blocks_for(n, 256)), dim3(256), 0, s, p, phase, force[0], force[1],
+                       force[2], cells, slots, n, out);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t d3q19_max_speed(const Lattice3<typename T::R>& p, int phase, const double force[3], double* partial,
+                           double* result, hipStream_t s) {
+    const std::uint32_t nc = blocks_for(p.cells, 256);
+    hipLaunchKernelGGL((d3q19_speed_cells<T>), dim3(nc), dim3(256), 0, s, p, phase, force[0], force[1], force[2],
+                       partial);
+    std::uint32_t n = nc;
+    if (p.nb > 0) {
+        const std::uint32_t nb = blocks_for(p.nb, 256);
+        hipLaunchKernelGGL((d3q19_speed_boundary<T>), dim3(nb), dim3(256), 0, s, p, phase, force[0], force[1],
+                           force[2], partial + nc);
+        n += nb;
+    }
+    hipLaunchKernelGGL(reduce_max, dim3(1), dim3(256), 0, s, partial, n, result);
+    return hipGetLastError();
+}
+
 std::uint32_t reduction_blocks(std::uint64_t n) { return blocks_for(n > 0 ? n : 1, 256); }
 
 template <class T>
@@ -671,6 +818,10 @@ hipError_t d3q19_force_sum(const Lattice3<typename T::R>& p, int phase, const st
                                             std::uint64_t, double, hipStream_t);                             \
     template hipError_t d3q19_populations<T>(const Lattice3<T::R>&, int, double*, hipStream_t);              \
     template hipError_t d3q19_moments<T>(const Lattice3<T::R>&, int, const double*, double*, hipStream_t);   \
+    template hipError_t d3q19_sample<T>(const Lattice3<T::R>&, int, const double*, const std::uint32_t*,      \
+                                        const std::int32_t*, std::uint32_t, double*, hipStream_t);            \
+    template hipError_t d3q19_max_speed<T>(const Lattice3<T::R>&, int, const double*, double*, double*,       \
+                                           hipStream_t);                                                      \
     template hipError_t d3q19_mass<T>(const Lattice3<T::R>&, int, double*, double*, hipStream_t);            \
     template hipError_t d3q19_force_sum<T>(const Lattice3<T::R>&, int, const std::uint32_t*,                 \
                                            const std::uint8_t*, const double*, const std::uint32_t*,         \

@@ -31,6 +31,14 @@ hipError_t d3q19_populations(const Lattice3<typename T::R>& p, int phase, double
 template <class T>
 hipError_t d3q19_moments(const Lattice3<typename T::R>& p, int phase, const double force[3], double* out,
                          hipStream_t s);
+// Moments of n listed nodes into out[4][n]; slots[k] is the boundary slot of cells[k] or -1.
+template <class T>
+hipError_t d3q19_sample(const Lattice3<typename T::R>& p, int phase, const double force[3], const std::uint32_t* cells,
+                        const std::int32_t* slots, std::uint32_t n, double* out, hipStream_t s);
+// Largest |u| over fluid nodes into *result (partial: reduction_blocks(cells) + reduction_blocks(nb)).
+template <class T>
+hipError_t d3q19_max_speed(const Lattice3<typename T::R>& p, int phase, const double force[3], double* partial,
+                           double* result, hipStream_t s);
 // partial: at least reduction_blocks(cells) + reduction_blocks(nb) doubles; out: 1 double.
 template <class T>
 hipError_t d3q19_mass(const Lattice3<typename T::R>& p, int phase, double* partial, double* out, hipStream_t s);
