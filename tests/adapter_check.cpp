@@ -5,12 +5,45 @@
 #include <cstring>
 #include <string>
 
+#include "orrery/geom/primitives.h"
+#include "orrery_lbm_aero.hpp"
 #include "orrery_lbm_solver.hpp"
 
 namespace ref = orrery::aero::lbm;
 
 int main(int argc, char** argv) {
     for (int a = 1; a < argc; ++a) {
+        if (std::strncmp(argv[a], "aero-", 5) == 0) {
+            // The IAeroSolver wrapper: a unit cube, a refused flow condition or option.
+            orrery::aero::AeroGeometry g;
+            g.mesh.vertices = {{-0.5, -0.5, -0.5}, {0.5, -0.5, -0.5}, {-0.5, 0.5, -0.5}, {0.5, 0.5, -0.5},
+                               {-0.5, -0.5, 0.5},  {0.5, -0.5, 0.5},  {-0.5, 0.5, 0.5},  {0.5, 0.5, 0.5}};
+            g.mesh.triangles = {{0, 4, 6}, {0, 6, 2}, {1, 3, 7}, {1, 7, 5}, {0, 1, 5}, {0, 5, 4},
+                                {2, 6, 7}, {2, 7, 3}, {0, 2, 3}, {0, 3, 1}, {4, 5, 7}, {4, 7, 6}};
+            g.reference.length = 1.0;
+            g.reference.area = 1.0;
+            orrery::aero::FlowConditions f;
+            f.speed = 0.1;
+            f.density = 1000.0;
+            f.viscosity = 5.0;
+            ref::LbmSolver3DOptions o;
+            const std::string what = argv[a] + 5;
+            if (what == "speed") {
+                f.speed = 0.0;
+            } else if (what == "sides") {
+                o.sides = ref::WallKind::no_slip;
+            } else if (what == "mach") {
+                o.resolution.lattice_velocity = 0.06;
+            }
+            orrery_amd::HipLbmAeroSolver3D solver(o);
+            try {
+                solver.setup(g, f);
+                std::printf("%s\tset up\n", argv[a]);
+            } catch (const std::invalid_argument& e) {
+                std::printf("%s\t%s\n", argv[a], e.what());
+            }
+            continue;
+        }
         ref::Lbm3DConfig c;
         c.nx = 16;
         c.ny = 8;

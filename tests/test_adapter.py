@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 REFERENCE = "/root/reference"
 INCLUDES = ["src/aero/lbm3d/include", "src/aero/lbm/include", "src/aero/compute/include", "src/core/include",
-            "src/geom/include"]
+            "src/geom/include", "src/aero/include"]
 
 pytestmark = pytest.mark.skipif(not (os.path.isdir(REFERENCE) and shutil.which("g++") and refcheck.available()),
                                 reason="needs the reference's headers and oracle/_ref")
@@ -61,3 +61,16 @@ def test_refusals_are_the_references(driver, what):
 def test_mask_shape_is_checked_before_the_pointer_crosses(driver):
     out = subprocess.run([driver, "size"], check=True, capture_output=True, text=True).stdout.strip()
     assert out == "size\tlbm3d: the solid mask does not match the lattice size"
+
+
+@pytest.mark.parametrize("what", ["speed", "sides", "mach"])
+def test_aero_solver_wrapper_refuses_like_the_adapter(driver, what):
+    """HipLbmAeroSolver3D (include/orrery_lbm_aero.hpp) behind the reference's IAeroSolver: its
+    setup refusals are the adapter's (the reference's texts)."""
+    out = subprocess.run([driver, "aero-" + what], check=True, capture_output=True, text=True).stdout.strip()
+    name, message = out.split("\t")
+    want = {"speed": "lbm3d: speed, density and viscosity must be positive and finite, angles finite",
+            "sides": "lbm3d: the channel sides are free slip or periodic",
+            "mach": "lattice units: lattice Mach number 0.103923 (u_lat 0.06) is not below 0.1; lower the lattice "
+                    "velocity (weak compressibility, APPROXIMATION A3)"}[what]
+    assert name == "aero-" + what and message == want
