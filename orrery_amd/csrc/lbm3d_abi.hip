@@ -193,6 +193,7 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     }
     h->cfg = c;
     h->storage = storage;
+    h->cpt = lbm::internal::default_cells_per_thread(storage);
     h->cells = cells;
     h->stride = stride;
     h->solid.assign(solid, solid + cells);

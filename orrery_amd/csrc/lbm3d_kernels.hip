@@ -18,115 +18,10 @@ using d3::off3;
 // Bulk: every bulk node (all 18 neighbours inside the domain and fluid) streams plainly, V
 // consecutive cells per thread. Boundary: one thread per boundary node. kLocal selects the A
 // step (odd step count) or the B step (even).
-// V consecutive stored populations of one plane, as one memory access.
-template <class T, int V> __device__ __forceinline__ void load_run(const typename T::S* src, typename T::R (&out)[V]) {
-    typename T::S tmp[V];
-    __builtin_memcpy(tmp, src, sizeof(tmp));
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-        out[k] = T::load(tmp[k]);
-    }
-}
-
-template <class T, int V> __device__ __forceinline__ void store_run(typename T::S* dst, const typename T::R (&in)[V]) {
-    typename T::S tmp[V];
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-        tmp[k] = T::pack(in[k]);
-    }
-    __builtin_memcpy(dst, tmp, sizeof(tmp));
-}
-
-// The bulk half of a step: each thread takes V consecutive cells and moves each population
-// plane as one V-wide access (19 loads and 19 stores per thread whatever V is). A run whose
-// cells are all bulk nodes stores whole runs; a mixed run stores its bulk cells one by one,
-// because the other cells' addresses belong to nodes the boundary kernel updates.
+// The bulk half of a step (d3::bulk_run in lbm3d_step.h): each thread takes V consecutive cells.
 template <class T, bool kLocal, int V>
 __global__ __launch_bounds__(256) void d3q19_bulk(Lattice3<typename T::R> p) {
-    using S = typename T::S;
-    using R = typename T::R;
-    const std::uint32_t c0 = (blockIdx.x * blockDim.x + threadIdx.x) * V;
-    if (c0 >= p.cells) {
-        return;
-    }
-    // The class array is padded with solid (0) entries up to the plane stride.
-    std::uint8_t kind[V];
-    __builtin_memcpy(kind, p.cls + c0, sizeof(kind));
-    bool any = false;
-    bool all = true;
-#pragma unroll
-    for (int c = 0; c < V; ++c) {
-        any = any || kind[c] == 1;
-        all = all && kind[c] == 1;
-    }
-    if (!any) {
-        return;
-    }
-    S* __restrict__ f = static_cast<S*>(p.f);
-    const std::uint32_t stride = p.stride;
-    const int nx = static_cast<int>(p.nx);
-    const int plane = static_cast<int>(p.nx * p.ny);
-    // B-step sources of a run reach nx ny + nx + 1 cells back (and V - 1 + that forward): the
-    // guard cells around the planes keep them in the allocation.
-    const std::uint32_t g = p.nx * p.ny + p.nx + 1U;
-    R v[V][19];
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        R run[V];
-        if constexpr (kLocal) {
-            load_run<T, V>(f + i * stride + c0, run);
-        } else {
-            load_run<T, V>(f - g + (L3::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(c0 + g) - d3::off3(i, nx, plane))), run);
-        }
-#pragma unroll
-        for (int c = 0; c < V; ++c) {
-            v[c][i] = run[c];
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < V; ++c) {
-        if (kind[c] == 1) {
-            R sponge = 0;
-            if (p.sponge_strength > R(0)) {
-                const std::uint32_t cell = c0 + c;
-                const std::uint32_t z = p.div_plane.div(cell);
-                const std::uint32_t rest = cell - z * static_cast<std::uint32_t>(plane);
-                const std::uint32_t y = p.div_nx.div(rest);
-                sponge = d3::sponge_sigma3(p, rest - y * p.nx, y, z);
-            }
-            d3::collide3<T>(v[c], p, sponge);
-        }
-    }
-    if (all) {
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            R run[V];
-#pragma unroll
-            for (int c = 0; c < V; ++c) {
-                run[c] = v[c][i];
-            }
-            if constexpr (kLocal) {
-                store_run<T, V>(f + L3::opp(i) * stride + c0, run);
-            } else {
-                store_run<T, V>(f + (i * stride + static_cast<std::uint32_t>(static_cast<int>(c0) + d3::off3(i, nx, plane))), run);
-            }
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < V; ++c) {
-            if (kind[c] == 1) {
-                const std::uint32_t cell = c0 + c;
-#pragma unroll
-                for (int i = 0; i < 19; ++i) {
-                    if constexpr (kLocal) {
-                        f[L3::opp(i) * stride + cell] = T::pack(v[c][i]);
-                    } else {
-                        f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + d3::off3(i, nx, plane))] = T::pack(v[c][i]);
-                    }
-                }
-            }
-        }
-    }
+    d3::bulk_run<T, kLocal, V>(p, (blockIdx.x * blockDim.x + threadIdx.x) * V);
 }
 
 // The whole step in one launch, in cell order: each wave takes 64 consecutive cells, runs the

@@ -353,6 +353,129 @@ __device__ __forceinline__ void bulk_node(const Lattice3<typename T::R>& p, std:
     bulk_finish<T, kLocal>(p, cell, v);
 }
 
+// V consecutive stored populations of one plane, as one memory access.
+template <class T, int V> __device__ __forceinline__ void load_run(const typename T::S* src, typename T::R (&out)[V]) {
+    typename T::S tmp[V];
+    __builtin_memcpy(tmp, src, sizeof(tmp));
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        out[k] = T::load(tmp[k]);
+    }
+}
+
+template <class T, int V> __device__ __forceinline__ void store_run(typename T::S* dst, const typename T::R (&in)[V]) {
+    typename T::S tmp[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        tmp[k] = T::pack(in[k]);
+    }
+    __builtin_memcpy(dst, tmp, sizeof(tmp));
+}
+
+// The bulk half of a step for the run of V cells starting at c0: each thread takes V
+// consecutive cells and moves each population
+// plane as one V-wide access (19 loads and 19 stores per thread whatever V is). A run whose
+// cells are all bulk nodes stores whole runs; a mixed run stores its bulk cells one by one,
+// because the other cells' addresses belong to nodes the boundary kernel updates.
+// Element `index` of a population plane: a uniform 64-bit plane base plus a 32-bit byte offset,
+// the addressing of the scalar-base memory instructions (one offset register serves every
+// plane of an A step). Byte offsets stay below 2^32: cells and guards are below 2^28.
+template <class S> __device__ __forceinline__ S* element(S* base, std::uint32_t index) {
+    using Byte = std::conditional_t<std::is_const_v<S>, const char, char>;
+    return reinterpret_cast<S*>(reinterpret_cast<Byte*>(base) + index * static_cast<std::uint32_t>(sizeof(S)));
+}
+
+template <class T, bool kLocal, int V>
+__device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::uint32_t c0) {
+    using S = typename T::S;
+    using R = typename T::R;
+    if (c0 >= p.cells) {
+        return;
+    }
+    // The class array is padded with solid (0) entries up to the plane stride.
+    std::uint8_t kind[V];
+    __builtin_memcpy(kind, p.cls + c0, sizeof(kind));
+    bool any = false;
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < V; ++c) {
+        any = any || kind[c] == 1;
+        all = all && kind[c] == 1;
+    }
+    if (!any) {
+        return;
+    }
+    S* __restrict__ f = static_cast<S*>(p.f);
+    const std::uint32_t stride = p.stride;
+    const int nx = static_cast<int>(p.nx);
+    const int plane = static_cast<int>(p.nx * p.ny);
+    // B-step sources of a run reach nx ny + nx + 1 cells back (and V - 1 + that forward): the
+    // guard cells around the planes keep them in the allocation.
+    const std::uint32_t g = p.nx * p.ny + p.nx + 1U;
+    R v[V][19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        R run[V];
+        if constexpr (kLocal) {
+            load_run<T, V>(element(f + static_cast<std::size_t>(i) * stride, c0), run);
+        } else {
+            load_run<T, V>(element(f + static_cast<std::size_t>(L3::opp(i)) * stride - g,
+                                   static_cast<std::uint32_t>(static_cast<int>(c0 + g) - off3(i, nx, plane))),
+                           run);
+        }
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            v[c][i] = run[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < V; ++c) {
+        if (kind[c] == 1) {
+            R sponge = 0;
+            if (p.sponge_strength > R(0)) {
+                const std::uint32_t cell = c0 + c;
+                const std::uint32_t z = p.div_plane.div(cell);
+                const std::uint32_t rest = cell - z * static_cast<std::uint32_t>(plane);
+                const std::uint32_t y = p.div_nx.div(rest);
+                sponge = sponge_sigma3(p, rest - y * p.nx, y, z);
+            }
+            collide3<T>(v[c], p, sponge);
+        }
+    }
+    if (all) {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            R run[V];
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                run[c] = v[c][i];
+            }
+            if constexpr (kLocal) {
+                store_run<T, V>(element(f + static_cast<std::size_t>(L3::opp(i)) * stride, c0), run);
+            } else {
+                store_run<T, V>(element(f + static_cast<std::size_t>(i) * stride,
+                                        static_cast<std::uint32_t>(static_cast<int>(c0) + off3(i, nx, plane))),
+                                run);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            if (kind[c] == 1) {
+                const std::uint32_t cell = c0 + c;
+#pragma unroll
+                for (int i = 0; i < 19; ++i) {
+                    if constexpr (kLocal) {
+                        f[L3::opp(i) * stride + cell] = T::pack(v[c][i]);
+                    } else {
+                        f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] = T::pack(v[c][i]);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Direction i after the mirrors a source code asks for, chosen among compile-time constants
 // (i is a constant of an unrolled loop) rather than read from a table at run time.
 __device__ __forceinline__ std::uint32_t mirrored(int i, std::uint32_t code) {

@@ -56,28 +56,17 @@ __device__ __forceinline__ std::uint32_t owner(const std::uint32_t* __restrict__
     return lo;
 }
 
-template <class T, int CPT>
+template <class T, int V>
 __global__ __launch_bounds__(256) void batch_bulk(const Lattice3<typename T::R>* __restrict__ lat,
                                                   const std::uint32_t* __restrict__ start, std::uint32_t n,
                                                   const std::uint32_t* __restrict__ parity) {
-    const std::uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-    const std::uint32_t l = owner(start, n, tile);
+    const std::uint32_t l = owner(start, n, blockIdx.x);
     const Lattice3<typename T::R>& p = lat[l];
-    const std::uint32_t local = tile - start[l];
-    const bool odd = (parity[l] & 1U) != 0;
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const std::uint32_t cell = (local * CPT + k) * blockDim.x + threadIdx.x;
-        if (cell >= p.cells) {
-            return;
-        }
-        if (p.cls[cell] == 1) {
-            if (odd) {
-                d3::bulk_node<T, true>(p, cell);
-            } else {
-                d3::bulk_node<T, false>(p, cell);
-            }
-        }
+    const std::uint32_t c0 = ((blockIdx.x - start[l]) * blockDim.x + threadIdx.x) * V;
+    if ((parity[l] & 1U) != 0) {
+        d3::bulk_run<T, true, V>(p, c0);
+    } else {
+        d3::bulk_run<T, false, V>(p, c0);
     }
 }
 
@@ -410,6 +399,7 @@ int rebuild(lbm_batch& b) {
         const int st = lbm::internal::storage_of(*b.entries[k].h);
         Group& g = by_store[st];
         g.storage = st;
+        g.cpt = lbm::internal::default_cells_per_thread(st);
         g.members.push_back(static_cast<int>(k));
     }
     for (auto& [st, g] : by_store) {

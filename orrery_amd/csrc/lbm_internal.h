@@ -71,6 +71,12 @@ inline std::size_t store_bytes(int storage) {
 
 inline std::size_t real_bytes(int storage) { return storage == LBM_STORE_F64 ? 8 : 4; }
 
+// Default cells per thread of the bulk kernel: runs of two 16-bit populations move as one
+// 32-bit access; binary32 and binary64 runs of one cell keep the most waves resident.
+inline int default_cells_per_thread(int storage) {
+    return storage == LBM_STORE_F16S || storage == LBM_STORE_BF16S ? 2 : 1;
+}
+
 // Guard cells before and after the population planes: the B step of the bulk kernel loads the
 // sources of every cell of its tile before it knows whether the cell is a bulk node, and those
 // sources lie at most nx ny + nx + 1 cells away. Rounded to 128 cells to keep the planes aligned.
