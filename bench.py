@@ -116,6 +116,56 @@ def measured_traffic():
         return json.load(f)
 
 
+def log(message):
+    print(f"bench: {message}", file=sys.stderr, flush=True)
+
+
+def polar_bench(args):
+    """Batched Cd solves per hour: the wing-section sweep in one batch against the same cases one
+    lattice at a time (host-paced force reads), same kernels, same convergence test."""
+    from orrery_amd import polar
+    cases = polar.wing_cases(args.lattices)
+    log(f"polar: voxelising {len(cases)} wing sections")
+    lattices = [polar.wing_lattice(c) for c in cases]
+    log("polar: batched run")
+    batched, tb = polar.run_batched(cases, lattices, log=log)
+    pick = list(range(0, len(cases), max(1, args.sequential_every)))
+    log(f"polar: one-at-a-time run of {len(pick)} cases")
+    sequential, ts = polar.run_sequential([cases[k] for k in pick], [lattices[k] for k in pick], log=log)
+    # The one-at-a-time rate on the cases it ran, scaled by their share of the batched work
+    # (cell-steps), so a subset of cases gives a like-for-like estimate.
+    work = {s.case.index: s.case.cells * s.steps for s in batched}
+    share = sum(work[k] for k in pick) / sum(work.values())
+    seq_total = ts / share
+    by_case = {s.case.index: s for s in batched}
+    same = all((s.steps, s.cd, s.cl) == (by_case[s.case.index].steps, by_case[s.case.index].cd,
+                                         by_case[s.case.index].cl) for s in sequential)
+    rate = len(batched) / tb * 3600.0
+    line = {
+        "metric": "batched Cd solves per hour (wing-section sweep, convergence |dCd| < 1e-4 over 500 steps)",
+        "value": round(rate, 1), "unit": "solves/h", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(cases)} NACA 0012 sections, dims in [96^3, 320x160x160], chord 48-96 cells, "
+                               "AoA -4..18 deg, Re 500-2000, LES, interpolated wall, periodic span",
+                   "cells": sum(c.cells for c in cases), "parallelism": "single GPU"},
+        "batched_s": round(tb, 2), "sequential_s_estimated": round(seq_total, 2),
+        "sequential_cases_timed": len(pick), "speedup": round(seq_total / tb, 3),
+        "same_results_as_sequential": same,
+        "converged": sum(s.converged for s in batched),
+        "lift_slope_per_rad": round(polar.lift_slope(batched), 4),
+        "glups_batched": round(sum(work.values()) / tb / 1e9, 2),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def serve_bench(args):
+    from orrery_amd import serve
+    result = serve.serve_bench(args.requests, log=log)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +173,13 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--storages", default="f32,f16s,bf16s")
+    ap.add_argument("--workload", choices=["channel", "polar", "serve"], default="channel",
+                    help="channel: the 512x256x256 lattice (the headline line); polar: the batched wing-section "
+                         "sweep, batched against one at a time; serve: the continuous-batching server")
+    ap.add_argument("--lattices", type=int, default=48, help="polar: number of wing cases")
+    ap.add_argument("--sequential-every", type=int, default=1,
+                    help="polar: time the one-at-a-time baseline on every k-th case (k > 1 bounds the run)")
+    ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device with a plain copy kernel (3 launches), the "
                          "known-size reference for the HBM byte counters of a profiler run")
@@ -133,6 +190,10 @@ def main(argv=None):
               file=sys.stderr)
         return 2
 
+    if args.workload == "polar":
+        return polar_bench(args)
+    if args.workload == "serve":
+        return serve_bench(args)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
     config, mask = workload()

@@ -1,0 +1,201 @@
+"""An angle-of-attack sweep of NACA 0012 wing sections, solved as one batch of resident lattices
+or one lattice at a time: the batched Cd workload.
+
+Each case is an infinite-span section (the span runs through periodic y sides) of chord c
+cells in a channel with a velocity inlet, a convective outlet, slip z sides and an absorbing
+layer before the outlet, at its own Reynolds number and angle of attack. A case has converged
+when the mean drag coefficient of the last 50 samples (500 steps) differs from that of the 50
+before by less than 1e-4, after at least two flow-throughs. The lift slope is the least-squares
+slope of Cl against the angle of attack over the attached range, as in the reference's
+comparison of solvers (src/app/aero/aero_comparison.cpp:101-164).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import geometry as geo
+from .batch import LatticeBatch
+from .config import Lbm3DConfig, Outlet, Smagorinsky, SolidWall, Sponge, Wall, WallKind, XBoundary
+from .masks import SolidMask3D, fluid_mask_3d
+from .solver import Lbm3DSolver
+
+U_LATTICE = 0.05
+SAMPLE_EVERY = 10
+WINDOW = 50          # samples in a mean (500 steps)
+TOLERANCE = 1e-4     # |Cd(t) - Cd(t - 500)|
+MAX_STEPS = 20000
+
+
+@dataclasses.dataclass
+class WingCase:
+    index: int
+    nx: int
+    ny: int
+    nz: int
+    chord: int
+    aoa_deg: float
+    reynolds: float
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def min_steps(self) -> int:
+        """Two flow-throughs of the channel, in whole sampling windows."""
+        steps = int(math.ceil(2.0 * self.nx / U_LATTICE))
+        return -(-steps // (WINDOW * SAMPLE_EVERY)) * WINDOW * SAMPLE_EVERY
+
+
+def wing_cases(n: int = 48, seed: int = 2024, aoa_range=(-4.0, 18.0)) -> list[WingCase]:
+    """n cases with seeded dims uniform over [96^3, 320 x 160 x 160], chords uniform over
+    [48, 96] cells (at most nx / 3), Reynolds numbers uniform over [500, 2000], and angles of
+    attack evenly spaced over `aoa_range` degrees."""
+    rng = np.random.default_rng(seed)
+    angles = np.linspace(aoa_range[0], aoa_range[1], n) if n > 1 else np.array([aoa_range[0]])
+    cases = []
+    for k in range(n):
+        nx = int(rng.integers(96, 321))
+        ny = int(rng.integers(96, 161))
+        nz = int(rng.integers(96, 161))
+        chord = int(min(rng.integers(48, 97), nx // 3))
+        cases.append(WingCase(k, nx, ny, nz, chord, float(angles[k]), float(rng.uniform(500.0, 2000.0))))
+    return cases
+
+
+def place_body(mesh: geo.TriangleMesh, nx: int, ny: int, nz: int, interpolated: bool = True,
+               margin: int = 4) -> SolidMask3D:
+    """Voxelises `mesh` (lattice units: node (i, j, k) sits at (i, j, k)) over its bounding box
+    plus `margin` nodes, clipped to the lattice, and places it in a fluid mask; with
+    `interpolated`, the signed distance within three cells becomes the mask's distance."""
+    lo, hi = mesh.bounds()
+    dims = (nx, ny, nz)
+    first = [max(0, int(math.floor(lo[a])) - margin) for a in range(3)]
+    last = [min(dims[a] - 1, int(math.ceil(hi[a])) + margin) for a in range(3)]
+    count = [last[a] - first[a] + 1 for a in range(3)]
+    spec = geo.GridSpec(tuple(first[a] - 0.5 for a in range(3)), 1.0, *count)
+    vox = geo.voxelize(mesh, spec)
+    mask = fluid_mask_3d(nx, ny, nz)
+    geo.place_voxels(mask, vox, first)
+    if interpolated:
+        geo.place_distance(mask, geo.signed_distance_band(mesh, vox, 3.0), first)
+    return mask
+
+
+def wing_lattice(case: WingCase, interpolated: bool = True) -> tuple[Lbm3DConfig, SolidMask3D]:
+    c = case.chord
+    # A section longer than the periodic span, so no cell sees the end caps within the
+    # three-cell distance band.
+    wing = geo.naca_wing(0.0, 0.0, 0.12, float(c), float(case.ny + 16), max(16, 2 * c), 1)
+    pivot = (0.25 * c, 0.0, 0.0)
+    to = (round(0.3 * case.nx) + 0.25, 0.5 * (case.ny - 1), 0.5 * (case.nz - 1) + 0.25)
+    mesh = wing.transformed(geo.rotation_y(math.radians(case.aoa_deg)), pivot, to)
+    mask = place_body(mesh, case.nx, case.ny, case.nz, interpolated)
+    tau = 0.5 + 3.0 * U_LATTICE * c / case.reynolds
+    config = Lbm3DConfig(nx=case.nx, ny=case.ny, nz=case.nz, tau=tau, x_boundary=XBoundary.inflow_outflow,
+                         outlet=Outlet.convective, y_low=Wall(WallKind.periodic), y_high=Wall(WallKind.periodic),
+                         z_low=Wall(WallKind.slip), z_high=Wall(WallKind.slip), inlet_ux=U_LATTICE,
+                         smagorinsky=Smagorinsky(True, 0.1), sponge=Sponge(0.1, 0, case.nx // 10, 0),
+                         solid_wall=SolidWall.interpolated if interpolated else SolidWall.halfway)
+    return config, mask
+
+
+def coefficients(case: WingCase, force: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """(Cd, Cl) of lattice forces [n, >=3]: F / (rho u^2 c span / 2), span = ny."""
+    q = 0.5 * U_LATTICE * U_LATTICE * case.chord * case.ny
+    return force[:, 0] / q, force[:, 2] / q
+
+
+def converged(case: WingCase, time_steps: np.ndarray, cd: np.ndarray) -> bool:
+    if len(cd) < 2 * WINDOW or time_steps[-1] < case.min_steps:
+        return False
+    a = cd[-WINDOW:].mean()
+    b = cd[-2 * WINDOW:-WINDOW].mean()
+    return bool(abs(a - b) < TOLERANCE)
+
+
+@dataclasses.dataclass
+class Solve:
+    case: WingCase
+    steps: int
+    cd: float
+    cl: float
+    converged: bool
+    finished_at: float  # seconds from the start of the run
+
+
+def run_batched(cases: list[WingCase], lattices, storage: str = "f32", budget_bytes: int = 240 * 10 ** 9,
+                chunk: int = WINDOW * SAMPLE_EVERY, seed: int = 7, log=None) -> tuple[list[Solve], float]:
+    """Every case resident in one batch (admission included in the time), advanced `chunk`
+    steps per launch sequence; converged cases leave the batch. `lattices` holds the
+    (config, mask) of each case. Returns the solves and the wall time."""
+    t0 = time.perf_counter()
+    batch = LatticeBatch(budget_bytes=budget_bytes, sample_every=SAMPLE_EVERY)
+    active = {}
+    for case, (config, mask) in zip(cases, lattices):
+        active[case.index] = (case, batch.add(config, mask, storage, seed, 1e-4))
+    done = []
+    while active:
+        batch.run(chunk)
+        for key in list(active):
+            case, lat = active[key]
+            t, f = lat.history()
+            cd, cl = coefficients(case, f)
+            ok = converged(case, t, cd)
+            if ok or t[-1] >= MAX_STEPS:
+                done.append(Solve(case, int(t[-1]), float(cd[-WINDOW:].mean()), float(cl[-WINDOW:].mean()), ok,
+                                  time.perf_counter() - t0))
+                batch.remove(lat)
+                del active[key]
+                if log:
+                    log(f"batched: case {case.index} done at step {int(t[-1])} ({len(active)} left)")
+    wall = time.perf_counter() - t0
+    batch.close()
+    return done, wall
+
+
+def run_sequential(cases: list[WingCase], lattices, storage: str = "f32", seed: int = 7,
+                   log=None) -> tuple[list[Solve], float]:
+    """One plain lattice at a time (its creation included in the time), its force read back
+    every sample (host-paced)."""
+    done = []
+    t0 = time.perf_counter()
+    for case, (config, mask) in zip(cases, lattices):
+        s = Lbm3DSolver(config, mask, storage)
+        s.initialize_perturbed(seed, 1e-4)
+        times = []
+        forces = []
+        while True:
+            s.step(SAMPLE_EVERY)
+            times.append(float(s.steps()))
+            forces.append(s.force().force)
+            if len(times) % WINDOW == 0:
+                t = np.array(times)
+                cd, cl = coefficients(case, np.array(forces))
+                ok = converged(case, t, cd)
+                if ok or t[-1] >= MAX_STEPS:
+                    done.append(Solve(case, int(t[-1]), float(cd[-WINDOW:].mean()), float(cl[-WINDOW:].mean()), ok,
+                                      time.perf_counter() - t0))
+                    if log:
+                        log(f"sequential: case {case.index} done at step {int(t[-1])}")
+                    break
+        s.close()
+    return done, time.perf_counter() - t0
+
+
+def lift_slope(solves: list[Solve], attached=(-4.0, 8.0)) -> float:
+    """Least-squares dCl/dalpha per radian over the cases with alpha in `attached` degrees."""
+    pts = [(math.radians(s.case.aoa_deg), s.cl) for s in solves if attached[0] <= s.case.aoa_deg <= attached[1]]
+    if len(pts) < 2:
+        return float("nan")
+    a = np.array(pts)
+    x = a[:, 0] - a[:, 0].mean()
+    return float((x * (a[:, 1] - a[:, 1].mean())).sum() / (x * x).sum())
+
+
+__all__ = ["WingCase", "wing_cases", "wing_lattice", "place_body", "run_batched", "run_sequential", "lift_slope",
+           "coefficients", "converged", "Solve"]
