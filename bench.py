@@ -3,13 +3,19 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Workload: one D3Q19 BGK + Smagorinsky lattice of 512 x 256 x 256 nodes (33.5 M) in a channel with
-a velocity inlet, a convective outlet and slip sides, around a voxelised body, FP32 populations
-stored in place (A-A), started from a seeded +-1e-4 perturbation of the uniform inlet state. A step
-is one lattice update of every node (bulk kernel + boundary kernel). `value` = nx ny nz K / time
-in GLUPS, timed with HIP events on the solver's stream after W warm-up steps. The roofline object
-prices the bulk kernel (the dominant one) at 152 algorithmic bytes per bulk node against MI355X's
-8 TB/s; cpu_baseline is the oracle (the FP64 CPU restatement of the reference solver) on a
-bounded number of steps of the same lattice on all host cores.
+a velocity inlet, a convective outlet and slip sides, around a voxelised Apollo-type capsule
+(diameter 64 cells, pitched 20 degrees, Re 1000), FP32 populations stored in place (A-A), started
+from a seeded +-1e-4 perturbation of the uniform inlet state. A step is one lattice update of
+every node (bulk kernel + boundary kernel). `value` = nx ny nz K / time in GLUPS, timed with HIP
+events on the solver's stream after W warm-up steps. The roofline object prices the bulk kernel
+(the dominant one) at 152 algorithmic bytes per bulk node against MI355X's 8 TB/s; cpu_baseline is
+the oracle (the FP64 CPU restatement of the reference solver) on a bounded number of steps of the
+same lattice on all host cores. Each storage format (f32, f16s, bf16s) is also run on to step
+2000 and its drag force and velocity field compared with the f64 storage's (the FP64 oracle
+arithmetic on the device) at that step.
+
+Other workloads (--workload): polar (the batched wing-section sweep), serve (the
+continuous-batching server), d2q9 (the 400 x 160 cylinder channel).
 """
 from __future__ import annotations
 
@@ -26,17 +32,28 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # bytes/s, MI355X published
 
 
-def workload(nx=512, ny=256, nz=256, diameter=64.0, reynolds=1000.0, u=0.05):
-    from orrery_amd import Lbm3DConfig, Smagorinsky, Wall, WallKind, XBoundary, sphere_mask, tau_of_viscosity
+def workload(nx=512, ny=256, nz=256, diameter=64.0, pitch_deg=20.0, reynolds=1000.0, u=0.05):
+    """The 512 x 256 x 256 channel around a voxelised Apollo-type capsule (heat-shield diameter 64
+    cells, heat shield facing the flow, pitched 20 degrees about y) at Re 1000 on the diameter."""
+    import math
+    from orrery_amd import Lbm3DConfig, Smagorinsky, Wall, WallKind, XBoundary, geometry, tau_of_viscosity
+    from orrery_amd.polar import place_body
+    from orrery_amd.serve import body_mesh
     config = Lbm3DConfig(nx=nx, ny=ny, nz=nz, tau=tau_of_viscosity(u * diameter / reynolds),
                          x_boundary=XBoundary.inflow_outflow, y_low=Wall(WallKind.slip), y_high=Wall(WallKind.slip),
                          z_low=Wall(WallKind.slip), z_high=Wall(WallKind.slip), inlet_ux=u,
                          smagorinsky=Smagorinsky(True, 0.17))
     centre = (2.0 * diameter, 0.5 * (ny - 1) + 0.25, 0.5 * (nz - 1) + 0.25)
-    return config, sphere_mask(nx, ny, nz, centre, 0.5 * diameter)
+    mesh = body_mesh("capsule", diameter).transformed(geometry.rotation_y(math.radians(pitch_deg)), (0.0, 0.0, 0.0),
+                                                       centre)
+    return config, place_body(mesh, nx, ny, nz, interpolated=False)
 
 
-def time_storage(config, mask, storage, steps, warmup, per_kernel_steps):
+def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare_steps=0, ref=None):
+    """Times `steps` steps after `warmup` (one event pair) and `per_kernel_steps` more per kernel,
+    then steps on to `compare_steps` and reads the force and velocity there: returned as the
+    reference fields for the f64 storage, compared with `ref` (the f64 run's) otherwise."""
+    import numpy as np
     from orrery_amd import Lbm3DSolver, abi
     import ctypes
     lib = abi.load()
@@ -57,9 +74,7 @@ def time_storage(config, mask, storage, steps, warmup, per_kernel_steps):
         raise RuntimeError(lib.lbm_last_error().decode())
     counts = [ctypes.c_size_t() for _ in range(4)]
     lib.lbm3d_counts(solver.handle, *[ctypes.byref(c) for c in counts])
-    fx = solver.force().force[0]
-    solver.close()
-    return {
+    out = {
         "ms_per_step": total_ms / steps,
         "bulk_ms": bulk.value / per_kernel_steps,
         "boundary_ms": bnd.value / per_kernel_steps,
@@ -67,8 +82,24 @@ def time_storage(config, mask, storage, steps, warmup, per_kernel_steps):
         "boundary_nodes": counts[1].value,
         "solid_nodes": counts[2].value,
         "links": counts[3].value,
-        "force_x": fx,
     }
+    if compare_steps > 0:
+        solver.step(max(0, compare_steps - solver.steps()))
+        fx = solver.force().force[0]
+        m = solver.macroscopic()
+        fluid = np.asarray(m.solid) == 0
+        velocity = np.stack([m.velocity_x[fluid], m.velocity_y[fluid], m.velocity_z[fluid]])
+        del m
+        out["compared_at_step"] = solver.steps()
+        out["force_x"] = fx
+        if ref is None:
+            out["fields"] = velocity
+        else:
+            out["force_x_rel_error_vs_f64"] = abs(fx / ref["force_x"] - 1.0)
+            out["velocity_l2_vs_f64"] = float(np.sqrt(((velocity - ref["fields"]) ** 2).sum() /
+                                                      (ref["fields"] ** 2).sum()))
+    solver.close()
+    return out
 
 
 def cpu_baseline(config, mask, steps):
@@ -159,6 +190,111 @@ def polar_bench(args):
     return 0
 
 
+D2Q9_SAMPLE = 10
+
+
+def d2q9_channel():
+    """400 x 160 channel, cylinder D 32 centred at (80, 80), inlet u 0.05, Re 100, zero-gradient
+    outlet, slip top and bottom, halfway bounce-back."""
+    from orrery_amd import Lbm2DConfig, Outlet, Wall, WallKind, XBoundary, circle_mask, tau_of_viscosity
+    u, d, re = 0.05, 32.0, 100.0
+    config = Lbm2DConfig(nx=400, ny=160, tau=tau_of_viscosity(u * d / re), x_boundary=XBoundary.inflow_outflow,
+                         outlet=Outlet.extrapolation, bottom=Wall(WallKind.slip), top=Wall(WallKind.slip), inlet_ux=u)
+    return config, circle_mask(400, 160, 80.0, 80.0, 0.5 * d), u, d
+
+
+def d2q9_run(solver, steps, u, d):
+    """Steps with a force sample every D2Q9_SAMPLE steps; returns (time, Cd, Cl) and the wall time."""
+    import numpy as np
+    q = 0.5 * u * u * d
+    t, cd, cl = [], [], []
+    t0 = time.perf_counter()
+    for k in range(steps // D2Q9_SAMPLE):
+        solver.step(D2Q9_SAMPLE)
+        f = solver.force()
+        t.append(float((k + 1) * D2Q9_SAMPLE))
+        cd.append(f.x / q)
+        cl.append(f.y / q)
+    return (np.array(t), np.array(cd), np.array(cl)), time.perf_counter() - t0
+
+
+def d2q9_summary(series, u, d, steps):
+    """Cd mean and Cl rms over the second half of the configured steps; the Strouhal number of the
+    lift oscillation after them (None when no settled oscillation is found)."""
+    import numpy as np
+    from orrery_amd import history
+    t, cd, cl = series
+    half = t <= steps
+    late = (t > steps // 2) & half
+    osc = history.dominant_oscillation(t, cl, float(steps))
+    return {"cd": float(history.mean_since(t[half], cd[half], steps // 2)),
+            "cl_rms": float(np.sqrt((cl[late] ** 2).mean())),
+            "strouhal": float(osc.frequency * d / u) if osc else None,
+            "shedding_cycles": int(osc.cycles) if osc else 0}
+
+
+def d2q9_bench(args):
+    """The plain D2Q9 cylinder channel on the device (FP32 and FP64 storage) and on the host
+    (the oracle), from the same seeded start, with a force sample every 10 steps."""
+    from orrery_amd import Lbm2DSolver
+    config, mask, u, d = d2q9_channel()
+    steps = args.d2q9_steps
+    total = 2 * steps  # the Strouhal number is measured after the configured steps
+    cells = config.nx * config.ny
+    out = {}
+    for storage in ("f32", "f64"):
+        s = Lbm2DSolver(config, mask, storage)
+        s.initialize_perturbed(2024, 1e-4)
+        log(f"d2q9: {storage}, {total} steps with a force sample every {D2Q9_SAMPLE}")
+        series, wall = d2q9_run(s, total, u, d)
+        out[storage] = dict(d2q9_summary(series, u, d, steps), mlups_sampled=cells * total / wall / 1e6,
+                            wall_s=wall)
+        # Stepping alone: graph replays against one launch pair per step.
+        for graphs in (True, False):
+            s.set_graphs(graphs)
+            s.step(args.steps)  # captures the graph the timed call replays
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(args.steps)
+            s.synchronize()
+            out[storage]["mlups_graphs" if graphs else "mlups_eager"] = cells * args.steps / (
+                time.perf_counter() - t0) / 1e6
+        s.close()
+    cpu = None
+    if args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle import Oracle2D, load
+        threads = os.cpu_count() or 1
+        load().orc_set_threads(threads)
+        o = Oracle2D(config, mask)
+        o.initialize_perturbed(2024, 1e-4)
+        log(f"d2q9: the oracle, {total} steps on {threads} threads")
+        series, wall = d2q9_run(o, total, u, d)
+        ref = d2q9_summary(series, u, d, steps)
+        cpu = {"value": round(cells * total / wall / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+               "sample": f"the whole run: {total} FP64 steps of the 400x160 channel with force samples ({wall:.1f} s)",
+               **{k: v for k, v in ref.items()}}
+        for r in out.values():
+            r["cd_rel_error_vs_oracle"] = abs(r["cd"] / ref["cd"] - 1.0)
+            if r["strouhal"] is not None and ref["strouhal"] is not None:
+                r["strouhal_rel_error_vs_oracle"] = abs(r["strouhal"] / ref["strouhal"] - 1.0)
+    f32 = out["f32"]
+    line = {
+        "metric": "D2Q9 MLUPS (cylinder channel, force sampled every 10 steps) with Cd, Cl and Strouhal number",
+        "value": round(f32["mlups_sampled"], 2), "unit": "MLUPS", "n_gpus": 1, "steps": total, "warmup": 0,
+        "ms_per_step": round(f32["wall_s"] / total * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "D2Q9 400x160 channel, cylinder D32 at (80,80), Re 100, u 0.05, zero-gradient outlet, "
+                               f"halfway walls, seeded 1e-4 perturbation; Cd and Cl over steps {steps // 2}-{steps}, "
+                               f"Strouhal number over steps {steps}-{total}", "parallelism": "single GPU"},
+        "storages": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                     for k, v in out.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def serve_bench(args):
     from orrery_amd import serve
     result = serve.serve_bench(args.requests, log=log)
@@ -173,9 +309,14 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--storages", default="f32,f16s,bf16s")
-    ap.add_argument("--workload", choices=["channel", "polar", "serve"], default="channel",
+    ap.add_argument("--compare-steps", type=int, default=2000,
+                    help="step at which every storage's force and velocity are compared with the f64 storage's "
+                         "(0: no comparison, no f64 run)")
+    ap.add_argument("--workload", choices=["channel", "polar", "serve", "d2q9"], default="channel",
                     help="channel: the 512x256x256 lattice (the headline line); polar: the batched wing-section "
-                         "sweep, batched against one at a time; serve: the continuous-batching server")
+                         "sweep, batched against one at a time; serve: the continuous-batching server; d2q9: the "
+                         "400x160 cylinder channel")
+    ap.add_argument("--d2q9-steps", type=int, default=20000, help="d2q9: configured steps (the run is twice as long)")
     ap.add_argument("--lattices", type=int, default=48, help="polar: number of wing cases")
     ap.add_argument("--sequential-every", type=int, default=1,
                     help="polar: time the one-at-a-time baseline on every k-th case (k > 1 bounds the run)")
@@ -194,15 +335,27 @@ def main(argv=None):
         return polar_bench(args)
     if args.workload == "serve":
         return serve_bench(args)
+    if args.workload == "d2q9":
+        return d2q9_bench(args)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
     config, mask = workload()
     per_kernel_steps = min(200, max(20, args.steps // 5))
+    compare = max(args.compare_steps, args.warmup + args.steps + per_kernel_steps) if args.compare_steps else 0
+    storages = [k for k in args.storages.split(",") if k]
+    if compare and "f64" not in storages:
+        storages.insert(0, "f64")
+    storages.sort(key=lambda k: k != "f64")
     results = {}
-    for storage in args.storages.split(","):
-        print(f"bench: timing {storage} ({args.warmup} warm-up, {args.steps} timed steps)", file=sys.stderr,
-              flush=True)
-        results[storage] = time_storage(config, mask, storage, args.steps, args.warmup, per_kernel_steps)
+    ref = None
+    for storage in storages:
+        print(f"bench: timing {storage} ({args.warmup} warm-up, {args.steps} timed steps"
+              + (f", compared with f64 at step {compare})" if compare else ")"), file=sys.stderr, flush=True)
+        results[storage] = time_storage(config, mask, storage, args.steps, args.warmup, per_kernel_steps, compare,
+                                        ref)
+        if storage == "f64" and compare:
+            ref = {"force_x": results[storage]["force_x"], "fields": results[storage].pop("fields")}
+    ref = None
     main_r = results["f32"]
     cells = config.cells
     glups = cells / (main_r["ms_per_step"] * 1e-3) / 1e9
@@ -227,8 +380,9 @@ def main(argv=None):
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "D3Q19 512x256x256 channel, sphere D64 (Re 1000, u 0.05, Smagorinsky C 0.17), "
-                        "velocity inlet, convective outlet, slip sides, FP32 A-A in place",
+            "workload": "D3Q19 512x256x256 channel, voxelised Apollo-type capsule D64 pitched 20 deg (Re 1000, "
+                        "u 0.05, Smagorinsky C 0.17), velocity inlet, convective outlet, slip sides, halfway "
+                        "bounce-back, FP32 A-A in place",
             "cells": cells,
             "bulk_nodes": main_r["bulk_nodes"],
             "boundary_nodes": main_r["boundary_nodes"],
@@ -256,7 +410,9 @@ def main(argv=None):
                 "bytes_per_node": 2 * 19 * {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[k],
                 "hbm_frac_bulk": round(2 * 19 * {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[k] * r["bulk_nodes"]
                                        / (r["bulk_ms"] * 1e-3) / HBM_PEAK, 4),
-                "ms_per_step": round(r["ms_per_step"], 4)}
+                "ms_per_step": round(r["ms_per_step"], 4),
+                **{e: float(f"{r[e]:.3e}") for e in ("force_x_rel_error_vs_f64", "velocity_l2_vs_f64") if e in r},
+                **({"compared_at_step": r["compared_at_step"]} if "compared_at_step" in r else {})}
             for k, r in results.items()
         },
     }

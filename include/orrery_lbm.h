@@ -155,6 +155,12 @@ int lbm3d_set_launch(lbm3d* h, int32_t block, int32_t cells_per_thread);
  * give the same results bit for bit. */
 enum { LBM_SCHEDULE_FUSED = 0, LBM_SCHEDULE_SERIAL = 1, LBM_SCHEDULE_STREAMS = 2 };
 int lbm3d_set_schedule(lbm3d* h, int32_t schedule);
+/* Whether lbm3d_step replays captured HIP graphs (on by default; no reference counterpart). A
+ * call of at least 8 steps under the serial or fused schedule runs them as replays of one graph
+ * of up to 64 steps, captured again when the inlet, schedule or launch shape changes; the
+ * results are those of the same steps launched one by one, bit for bit. ORRERY_LBM_GRAPHS=0 in the
+ * environment makes off the default for handles created afterwards. */
+int lbm3d_set_graphs(lbm3d* h, int32_t enabled);
 /* Bytes one step of this lattice moves by the algorithmic count: 2 * 19 * bytes per stored
  * population, times nx * ny * nz (the reference's MLUPS convention). */
 int lbm3d_step_bytes(const lbm3d* h, double* bytes);
@@ -174,6 +180,8 @@ int lbm2d_initialize(lbm2d* h, const double* moments /* 3 planes: rho, ux, uy, o
 int lbm2d_initialize_perturbed(lbm2d* h, uint64_t seed, double amplitude);
 int lbm2d_step(lbm2d* h, uint64_t count);
 uint64_t lbm2d_steps(const lbm2d* h);
+/* As lbm3d_set_graphs, for lbm2d_step. */
+int lbm2d_set_graphs(lbm2d* h, int32_t enabled);
 int lbm2d_force(lbm2d* h, double* fx, double* fy, double* moment);
 int lbm2d_mass(lbm2d* h, double* mass);
 int lbm2d_macroscopic(lbm2d* h, double* rho, double* ux, double* uy);

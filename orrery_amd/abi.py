@@ -123,6 +123,7 @@ EXPORTS = [
     ("lbm3d_device_bytes", ctypes.c_int, [ctypes.c_void_p, c_sizep]),
     ("lbm3d_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("lbm3d_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("lbm3d_set_graphs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("lbm3d_set_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("lbm3d_step_bytes", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_counts", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep, c_sizep, c_sizep]),
@@ -133,6 +134,7 @@ EXPORTS = [
     ("lbm2d_initialize_perturbed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double]),
     ("lbm2d_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
     ("lbm2d_steps", ctypes.c_uint64, [ctypes.c_void_p]),
+    ("lbm2d_set_graphs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("lbm2d_force", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p]),
     ("lbm2d_mass", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm2d_macroscopic", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p]),

@@ -3,12 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "lbm_device.h"
+#include "lbm_graph.h"
 #include "lbm_host.h"
 #include "lbm_kernels.h"
 
@@ -36,6 +38,8 @@ struct lbm2d {
     double* partial = nullptr;
     double* result = nullptr;
     std::vector<void*> owned;
+    // Replayed runs of steps of lbm2d_step.
+    lbm::StepGraph graph;
 };
 
 namespace {
@@ -51,7 +55,8 @@ int hip_fail2(const char* what, hipError_t e) {
 
 template <class T> lbm::Lattice2<typename T::R> descriptor2(const lbm2d& h) {
     using R = typename T::R;
-    lbm::Lattice2<R> p{};
+    lbm::Lattice2<R> p;
+    std::memset(static_cast<void*>(&p), 0, sizeof(p));  // padding too: the bytes key step graphs
     const lbm2d_config& c = h.cfg;
     p.f = h.f;
     p.cls = h.cls;
@@ -288,17 +293,23 @@ int lbm2d_step(lbm2d* h, uint64_t count) {
     if (h == nullptr) {
         return fail(LBM_EINVAL, "lbm2d: null handle");
     }
-    for (std::uint64_t n = 0; n < count; ++n) {
-        const bool local = (h->steps & 1U) != 0;
-        const int rc = with_store2(*h, [&]<class T>() -> int {
-            const hipError_t e = lbm::d2q9_step<T>(descriptor2<T>(*h), local, h->block, h->stream);
-            return e == hipSuccess ? ok() : hip_fail2("step", e);
+    return with_store2(*h, [&]<class T>() -> int {
+        const auto p = descriptor2<T>(*h);
+        std::vector<char> key;
+        lbm::key_append(key, p);
+        lbm::key_append(key, h->block);
+        const hipError_t e = h->graph.run(h->stream, count, h->steps, key, [&](bool local) {
+            return lbm::d2q9_step<T>(p, local, h->block, h->stream);
         });
-        if (rc != LBM_OK) {
-            return rc;
-        }
-        ++h->steps;
+        return e == hipSuccess ? ok() : hip_fail2("step", e);
+    });
+}
+
+int lbm2d_set_graphs(lbm2d* h, int32_t enabled) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm2d: null handle");
     }
+    h->graph.enable(enabled != 0);
     return ok();
 }
 

@@ -337,17 +337,34 @@ int lbm3d_step(lbm3d* h, uint64_t count) {
     if (h == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null handle");
     }
-    for (std::uint64_t n = 0; n < count; ++n) {
-        const bool local = (h->steps & 1U) != 0;
-        const int rc = with_store(*h, [&]<class T>() -> int {
-            const hipError_t e = step_once<T>(*h, descriptor<T>(*h), local);
-            return e == hipSuccess ? ok() : hip_fail("step", e);
-        });
-        if (rc != LBM_OK) {
-            return rc;
+    return with_store(*h, [&]<class T>() -> int {
+        const auto p = descriptor<T>(*h);
+        const auto launch = [&](bool local) { return step_once<T>(*h, p, local); };
+        if (h->schedule == LBM_SCHEDULE_STREAMS) {
+            for (std::uint64_t n = 0; n < count; ++n) {
+                const hipError_t e = launch((h->steps & 1U) != 0);
+                if (e != hipSuccess) {
+                    return hip_fail("step", e);
+                }
+                ++h->steps;
+            }
+            return ok();
         }
-        ++h->steps;
+        std::vector<char> key;
+        lbm::key_append(key, p);
+        lbm::key_append(key, h->schedule);
+        lbm::key_append(key, h->block);
+        lbm::key_append(key, h->cpt);
+        const hipError_t e = h->graph.run(h->stream, count, h->steps, key, launch);
+        return e == hipSuccess ? ok() : hip_fail("step", e);
+    });
+}
+
+int lbm3d_set_graphs(lbm3d* h, int32_t enabled) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
     }
+    h->graph.enable(enabled != 0);
     return ok();
 }
 

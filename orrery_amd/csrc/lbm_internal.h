@@ -7,10 +7,12 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "../../include/orrery_lbm.h"
 #include "lbm_device.h"
+#include "lbm_graph.h"
 #include "lbm_host.h"
 #include "lbm_kernels.h"
 
@@ -53,6 +55,8 @@ struct lbm3d {
     double* result = nullptr;
     std::size_t device_bytes = 0;
     std::vector<void*> owned;
+    // Replayed runs of steps of lbm3d_step (serial and fused schedules).
+    lbm::StepGraph graph;
 };
 
 namespace lbm::internal {
@@ -116,7 +120,8 @@ inline std::size_t required_bytes3(const lbm3d_config& c, const std::uint8_t* so
 
 template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
     using R = typename T::R;
-    lbm::Lattice3<R> p{};
+    lbm::Lattice3<R> p;
+    std::memset(static_cast<void*>(&p), 0, sizeof(p));  // padding too: the bytes key step graphs
     const lbm3d_config& c = h.cfg;
     p.f = h.f;
     p.cls = h.cls;

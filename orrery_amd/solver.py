@@ -220,9 +220,13 @@ class Lbm3DSolver:
         _check(self._lib, self._lib.lbm3d_set_launch(self._h, block, cells_per_thread))
 
     def set_schedule(self, schedule: str) -> None:
-        """"fused" (one kernel per step, the default), "serial" (bulk then boundary kernel) or
+        """"serial" (bulk then boundary kernel, the default), "fused" (one kernel per step) or
         "streams" (the two kernels concurrently on two streams); all give identical results."""
         _check(self._lib, self._lib.lbm3d_set_schedule(self._h, abi.SCHEDULE[schedule]))
+
+    def set_graphs(self, enabled: bool) -> None:
+        """Whether step() replays captured HIP graphs (the default) or launches every step."""
+        _check(self._lib, self._lib.lbm3d_set_graphs(self._h, 1 if enabled else 0))
 
     def step_bytes(self) -> float:
         v = ctypes.c_double()
@@ -267,6 +271,10 @@ class Lbm2DSolver:
 
     def steps(self) -> int:
         return int(self._lib.lbm2d_steps(self._h))
+
+    def set_graphs(self, enabled: bool) -> None:
+        """Whether step() replays captured HIP graphs (the default) or launches every step."""
+        _check(self._lib, self._lib.lbm2d_set_graphs(self._h, 1 if enabled else 0))
 
     def initialize(self, initial=None) -> None:
         if initial is None:
