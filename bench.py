@@ -49,18 +49,31 @@ def workload(nx=512, ny=256, nz=256, diameter=64.0, pitch_deg=20.0, reynolds=100
     return config, place_body(mesh, nx, ny, nz, interpolated=False)
 
 
-def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare_steps=0, ref=None):
+def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare_steps=0, ref=None, tune_steps=0):
     """Times `steps` steps after `warmup` (one event pair) and `per_kernel_steps` more per kernel,
     then steps on to `compare_steps` and reads the force and velocity there: returned as the
-    reference fields for the f64 storage, compared with `ref` (the f64 run's) otherwise."""
+    reference fields for the f64 storage, compared with `ref` (the f64 run's) otherwise. With
+    `tune_steps`, the autotuner picks the schedule and launch shape after the warm-up (the
+    lattice is restored after tuning)."""
     import numpy as np
-    from orrery_amd import Lbm3DSolver, abi
+    from orrery_amd import Lbm3DSolver, abi, autotune
     import ctypes
     lib = abi.load()
     solver = Lbm3DSolver(config, mask, storage)
     solver.initialize_perturbed(2024, 1e-4)
     solver.step(warmup)
     solver.synchronize()
+    launch = None
+    if tune_steps > 0:
+        best, tried = autotune.tune(solver, tune_steps)
+        launch = {"schedule": best.schedule, "block": best.block, "cells_per_thread": best.cells_per_thread,
+                  "tuned_ms_per_step": round(best.ms_per_step, 4),
+                  "default_ms_per_step": round(next(t.ms_per_step for t in tried if t.schedule == "serial" and t.block == 256
+                                                    and t.cells_per_thread == (2 if storage in ("f16s", "bf16s") else 1)), 4),
+                  "candidates": len(tried)}
+    else:
+        sched, block, cpt = autotune.current(solver)
+        launch = {"schedule": sched, "block": block, "cells_per_thread": cpt}
     ms = ctypes.c_double()
     bulk = ctypes.c_double()
     bnd = ctypes.c_double()
@@ -75,6 +88,7 @@ def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare
     counts = [ctypes.c_size_t() for _ in range(4)]
     lib.lbm3d_counts(solver.handle, *[ctypes.byref(c) for c in counts])
     out = {
+        "launch": launch,
         "ms_per_step": total_ms / steps,
         "bulk_ms": bulk.value / per_kernel_steps,
         "boundary_ms": bnd.value / per_kernel_steps,
@@ -309,6 +323,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--storages", default="f32,f16s,bf16s")
+    ap.add_argument("--tune-steps", type=int, default=10,
+                    help="steps per candidate of the launch autotuner before timing (0: the default launch)")
     ap.add_argument("--compare-steps", type=int, default=2000,
                     help="step at which every storage's force and velocity are compared with the f64 storage's "
                          "(0: no comparison, no f64 run)")
@@ -352,7 +368,7 @@ def main(argv=None):
         print(f"bench: timing {storage} ({args.warmup} warm-up, {args.steps} timed steps"
               + (f", compared with f64 at step {compare})" if compare else ")"), file=sys.stderr, flush=True)
         results[storage] = time_storage(config, mask, storage, args.steps, args.warmup, per_kernel_steps, compare,
-                                        ref)
+                                        ref, args.tune_steps)
         if storage == "f64" and compare:
             ref = {"force_x": results[storage]["force_x"], "fields": results[storage].pop("fields")}
     ref = None
@@ -360,7 +376,12 @@ def main(argv=None):
     cells = config.cells
     glups = cells / (main_r["ms_per_step"] * 1e-3) / 1e9
     bytes_per_node = 2 * 19 * 4
-    achieved = bytes_per_node * main_r["bulk_nodes"] / (main_r["bulk_ms"] * 1e-3)
+    fused = main_r["launch"]["schedule"] == "fused"
+    if fused:
+        # one kernel per step: its time is the step time, over every node
+        achieved = bytes_per_node * cells / (main_r["ms_per_step"] * 1e-3)
+    else:
+        achieved = bytes_per_node * main_r["bulk_nodes"] / (main_r["bulk_ms"] * 1e-3)
     cpu = None
     if args.cpu_steps > 0:
         print("bench: timing the CPU baseline (oracle)", file=sys.stderr, flush=True)
@@ -399,7 +420,8 @@ def main(argv=None):
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "traffic_over_algorithmic": round(traffic["bytes_per_launch"] / (bytes_per_node * main_r["bulk_nodes"]), 4)
             if traffic else None,
-            "kernel": "d3q19_bulk<StoreF32>",
+            "kernel": "d3q19_fused<StoreF32>" if fused else "d3q19_bulk<StoreF32>",
+            "launch": main_r["launch"],
             "bulk_ms": round(main_r["bulk_ms"], 4),
             "boundary_ms": round(main_r["boundary_ms"], 4),
             "step_roofline_frac": round(bytes_per_node * cells / (main_r["ms_per_step"] * 1e-3) / HBM_PEAK, 4),
@@ -410,7 +432,7 @@ def main(argv=None):
                 "bytes_per_node": 2 * 19 * {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[k],
                 "hbm_frac_bulk": round(2 * 19 * {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[k] * r["bulk_nodes"]
                                        / (r["bulk_ms"] * 1e-3) / HBM_PEAK, 4),
-                "ms_per_step": round(r["ms_per_step"], 4),
+                "ms_per_step": round(r["ms_per_step"], 4), "launch": r["launch"],
                 **{e: float(f"{r[e]:.3e}") for e in ("force_x_rel_error_vs_f64", "velocity_l2_vs_f64") if e in r},
                 **({"compared_at_step": r["compared_at_step"]} if "compared_at_step" in r else {})}
             for k, r in results.items()

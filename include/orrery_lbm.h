@@ -94,6 +94,8 @@ const char* lbm_version(void);
 int lbm_device_memory(size_t* available, size_t* max_alloc);
 /* 0 clears the fake limit. */
 int lbm_set_memory_limit(size_t bytes);
+/* Name, architecture and compute-unit count of the current device (the autotuner's cache key). */
+int lbm_device_name(char* name, size_t capacity);
 
 /* ---- D3Q19 -------------------------------------------------------------------------------- */
 /* make_gpu_solver (src/aero/lbm3d/lbm3d_gpu.cpp:487-513): validate() -> 32-bit index bound ->
@@ -161,6 +163,24 @@ int lbm3d_set_schedule(lbm3d* h, int32_t schedule);
  * results are those of the same steps launched one by one, bit for bit. ORRERY_LBM_GRAPHS=0 in the
  * environment makes off the default for handles created afterwards. */
 int lbm3d_set_graphs(lbm3d* h, int32_t enabled);
+/* The schedule and bulk launch shape lbm3d_step uses. */
+int lbm3d_get_launch(const lbm3d* h, int32_t* schedule, int32_t* block, int32_t* cells_per_thread);
+/* One candidate of the autotuner: a schedule with a bulk launch shape, and its measured time. */
+typedef struct lbm_launch_choice {
+    int32_t schedule;          /* LBM_SCHEDULE_* */
+    int32_t block;             /* threads per block of the bulk kernel */
+    int32_t cells_per_thread;  /* consecutive cells per thread of the bulk kernel */
+    double ms_per_step;
+} lbm_launch_choice;
+/* Autotuner (no reference counterpart): times `steps` steps (rounded up to even) under every
+ * bulk launch shape (block 64/128/192/256 x 1/2/4 cells per thread) of the serial schedule, and
+ * the fused schedule unless serial_only, then keeps the fastest on the handle. The lattice is
+ * copied on the device first and restored afterwards, with its step count: its later results
+ * are those it would have had without tuning, bit for bit (all schedules and shapes compute
+ * the same values). Needs device memory for that copy (LBM_ENOMEM otherwise). `tried` (capacity
+ * entries, may be NULL) receives every candidate in the order timed; *count their number. */
+int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_choice* tried, size_t capacity,
+                   size_t* count, lbm_launch_choice* best);
 /* Bytes one step of this lattice moves by the algorithmic count: 2 * 19 * bytes per stored
  * population, times nx * ny * nz (the reference's MLUPS convention). */
 int lbm3d_step_bytes(const lbm3d* h, double* bytes);
@@ -222,6 +242,11 @@ int lbm_batch_history(lbm_batch* b, int32_t id, double* time, double* force, siz
 int lbm_batch_bytes(const lbm_batch* b, size_t* used, size_t* budget);
 /* Launch shape of the bulk kernel for the lattices of one storage format (autotuner). */
 int lbm_batch_set_launch(lbm_batch* b, int32_t storage, int32_t block, int32_t cells_per_thread);
+/* Autotunes the bulk launch shape of each storage format present in the batch on a scratch
+ * copy of its largest lattice (lbm3d_autotune, serial schedule; the batch's lattices are not
+ * advanced) and applies it with lbm_batch_set_launch. best[storage] (4 entries, indexed by
+ * LBM_STORE_*) receives the choice, block 0 for a storage the batch does not hold. */
+int lbm_batch_autotune(lbm_batch* b, uint32_t steps, lbm_launch_choice* best);
 /* set_inlet_velocity of one lattice of the batch. */
 int lbm_batch_set_inlet(lbm_batch* b, int32_t id, const double velocity[3]);
 

@@ -56,6 +56,11 @@ class COscillation(ctypes.Structure):
                 ("amplitude", ctypes.c_double), ("period_spread", ctypes.c_double)]
 
 
+class CLaunchChoice(ctypes.Structure):
+    _fields_ = [("schedule", ctypes.c_int32), ("block", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32),
+                ("ms_per_step", ctypes.c_double)]
+
+
 D = ctypes.c_double
 U64 = ctypes.c_uint64
 I32 = ctypes.c_int32
@@ -104,6 +109,7 @@ EXPORTS = [
     ("lbm_version", ctypes.c_char_p, []),
     ("lbm_device_memory", ctypes.c_int, [c_sizep, c_sizep]),
     ("lbm_set_memory_limit", ctypes.c_int, [ctypes.c_size_t]),
+    ("lbm_device_name", ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     ("lbm3d_create", ctypes.c_int, [P(CConfig3D), c_u8p, c_f64p, ctypes.c_int32, P(ctypes.c_void_p)]),
     ("lbm3d_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("lbm3d_initialize", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
@@ -124,6 +130,9 @@ EXPORTS = [
     ("lbm3d_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("lbm3d_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("lbm3d_set_graphs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("lbm3d_get_launch", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32)]),
+    ("lbm3d_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, P(CLaunchChoice),
+                                      ctypes.c_size_t, c_sizep, P(CLaunchChoice)]),
     ("lbm3d_set_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("lbm3d_step_bytes", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_counts", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep, c_sizep, c_sizep]),
@@ -161,6 +170,7 @@ EXPORTS = [
                                          c_sizep]),
     ("lbm_batch_bytes", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep]),
     ("lbm_batch_set_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    ("lbm_batch_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, P(CLaunchChoice)]),
     ("lbm_batch_set_inlet", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p]),
     ("lbm_force_coefficients", None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, c_f64p,
                                       c_f64p]),

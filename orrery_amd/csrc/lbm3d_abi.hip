@@ -740,4 +740,109 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
     return rc;
 }
 
+int lbm3d_get_launch(const lbm3d* h, int32_t* schedule, int32_t* block, int32_t* cells_per_thread) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    if (schedule != nullptr) *schedule = h->schedule;
+    if (block != nullptr) *block = h->block;
+    if (cells_per_thread != nullptr) *cells_per_thread = h->cpt;
+    return ok();
+}
+
+int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_choice* tried, size_t capacity,
+                   size_t* count, lbm_launch_choice* best) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    if (steps == 0) {
+        return fail(LBM_EINVAL, "lbm3d: autotune needs at least one timed step");
+    }
+    const std::uint64_t timed = steps + (steps & 1U); // whole A-B pairs
+    // The state a step reads and writes: populations (with their guard cells), the boundary
+    // stash and the outlet history. A device copy of it brings the lattice back after timing.
+    const std::size_t guard = lbm::internal::guard_cells(h->cfg.nx, h->cfg.ny);
+    const std::size_t sb = store_bytes(h->storage);
+    const std::size_t rb = real_bytes(h->storage);
+    const std::size_t f_bytes = (19U * static_cast<std::size_t>(h->stride) + 2U * guard) * sb;
+    const std::size_t stash_bytes = 2U * 19U * static_cast<std::size_t>(h->tables.bstride) * rb;
+    const std::size_t hist_bytes = 5U * static_cast<std::size_t>(h->tables.bstride) * rb;
+    char* f_base = static_cast<char*>(h->f) - guard * sb;
+    void* snap = nullptr;
+    if (hipMalloc(&snap, f_bytes + stash_bytes + hist_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(LBM_ENOMEM, "lbm3d: autotune needs a device copy of the lattice state (" +
+                                    lbm::host::megabytes(f_bytes + stash_bytes + hist_bytes) + ")");
+    }
+    char* s = static_cast<char*>(snap);
+    hipError_t e = hipMemcpyAsync(s, f_base, f_bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s + f_bytes, h->stash, stash_bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s + f_bytes + stash_bytes, h->hist, hist_bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(snap);
+        return hip_fail("autotune snapshot", e);
+    }
+    const int saved_schedule = h->schedule;
+    const int saved_block = h->block;
+    const int saved_cpt = h->cpt;
+    const std::uint64_t saved_steps = h->steps;
+    std::vector<lbm_launch_choice> cand;
+    for (const int block : {256, 192, 128, 64}) {
+        for (const int cpt : {1, 2, 4}) {
+            cand.push_back({LBM_SCHEDULE_SERIAL, block, cpt, 0.0});
+        }
+    }
+    if (!serial_only) {
+        cand.push_back({LBM_SCHEDULE_FUSED, 256, 1, 0.0});
+    }
+    int rc = LBM_OK;
+    for (auto& c : cand) {
+        h->schedule = c.schedule;
+        h->block = c.block;
+        h->cpt = c.cells_per_thread;
+        double ms = 0.0;
+        rc = lbm3d_time_steps(h, 2, 0, &ms, nullptr, nullptr); // warm-up
+        if (rc == LBM_OK) {
+            rc = lbm3d_time_steps(h, timed, 0, &ms, nullptr, nullptr);
+        }
+        if (rc != LBM_OK) {
+            break;
+        }
+        c.ms_per_step = ms / static_cast<double>(timed);
+    }
+    e = hipMemcpyAsync(f_base, s, f_bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->stash, s + f_bytes, stash_bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->hist, s + f_bytes + stash_bytes, hist_bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(snap);
+    h->steps = saved_steps;
+    h->graph.reset();
+    if (e != hipSuccess) {
+        return hip_fail("autotune restore", e);
+    }
+    if (rc != LBM_OK) {
+        h->schedule = saved_schedule;
+        h->block = saved_block;
+        h->cpt = saved_cpt;
+        return rc;
+    }
+    std::size_t k = 0;
+    for (std::size_t i = 1; i < cand.size(); ++i) {
+        if (cand[i].ms_per_step < cand[k].ms_per_step) {
+            k = i;
+        }
+    }
+    h->schedule = cand[k].schedule;
+    h->block = cand[k].block;
+    h->cpt = cand[k].cells_per_thread;
+    if (best != nullptr) *best = cand[k];
+    if (tried != nullptr) {
+        for (std::size_t i = 0; i < cand.size() && i < capacity; ++i) {
+            tried[i] = cand[i];
+        }
+    }
+    if (count != nullptr) *count = cand.size();
+    return ok();
+}
+
 } // extern "C"

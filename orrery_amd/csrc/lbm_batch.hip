@@ -686,6 +686,49 @@ int lbm_batch_set_launch(lbm_batch* b, int32_t storage, int32_t block, int32_t c
     return ok();
 }
 
+int lbm_batch_autotune(lbm_batch* b, uint32_t steps, lbm_launch_choice* best) {
+    if (b == nullptr || best == nullptr) {
+        return fail(LBM_EINVAL, "lbm batch: null argument");
+    }
+    if (b->stream != nullptr) {
+        const hipError_t e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) {
+            return hip_fail_b("autotune", e);
+        }
+    }
+    for (int st = 0; st < 4; ++st) {
+        best[st] = lbm_launch_choice{LBM_SCHEDULE_SERIAL, 0, 0, 0.0};
+        // The largest lattice of this storage stands for the group, copied into a scratch
+        // handle (halfway walls: the bulk kernel, which the shape tunes, is the same).
+        const lbm_batch::Entry* pick = nullptr;
+        for (const auto& e : b->entries) {
+            if (e.h->storage == st && (pick == nullptr || e.h->cells > pick->h->cells)) {
+                pick = &e;
+            }
+        }
+        if (pick == nullptr) {
+            continue;
+        }
+        lbm3d_config c = pick->h->cfg;
+        c.solid_wall = LBM_SOLID_HALFWAY;
+        lbm3d* scratch = nullptr;
+        int rc = lbm3d_create(&c, pick->h->solid.data(), nullptr, st, &scratch);
+        if (rc == LBM_OK) {
+            rc = lbm3d_autotune(scratch, steps, 1, nullptr, 0, nullptr, &best[st]);
+        }
+        if (scratch != nullptr) {
+            (void)lbm3d_destroy(scratch);
+        }
+        if (rc == LBM_OK) {
+            rc = lbm_batch_set_launch(b, st, best[st].block, best[st].cells_per_thread);
+        }
+        if (rc != LBM_OK) {
+            return rc;
+        }
+    }
+    return ok();
+}
+
 int lbm_batch_set_inlet(lbm_batch* b, int32_t id, const double velocity[3]) {
     if (b == nullptr) {
         return fail(LBM_EINVAL, "lbm batch: null handle");

@@ -2,6 +2,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
+#include <string>
+
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <iomanip>
@@ -529,6 +533,27 @@ const char* lbm_last_error(void) { return lbm::host::last_error(); }
 const char* lbm_version(void) { return "orrery_lbm 0.1 gfx950"; }
 
 int lbm_device_memory(size_t* available, size_t* max_alloc) { return lbm::host::device_memory(available, max_alloc); }
+
+int lbm_device_name(char* name, size_t capacity) {
+    if (name == nullptr || capacity == 0) {
+        return lbm::host::fail(LBM_EINVAL, "lbm: null argument");
+    }
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) {
+        e = hipGetDeviceProperties(&prop, dev);
+    }
+    if (e != hipSuccess) {
+        return lbm::host::fail(LBM_EDEVICE, std::string("lbm: device properties: ") + hipGetErrorString(e));
+    }
+    const std::string text = std::string(prop.name) + " " + prop.gcnArchName + " (" +
+                             std::to_string(prop.multiProcessorCount) + " CUs)";
+    const std::size_t n = std::min(capacity - 1, text.size());
+    std::memcpy(name, text.data(), n);
+    name[n] = '\0';
+    return LBM_OK;
+}
 
 int lbm_set_memory_limit(size_t bytes) {
     lbm::host::set_memory_limit(bytes);

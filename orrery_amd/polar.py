@@ -17,6 +17,7 @@ import time
 
 import numpy as np
 
+from . import autotune
 from . import geometry as geo
 from .batch import LatticeBatch
 from .config import Lbm3DConfig, Outlet, Smagorinsky, SolidWall, Sponge, Wall, WallKind, XBoundary
@@ -129,15 +130,21 @@ class Solve:
 
 
 def run_batched(cases: list[WingCase], lattices, storage: str = "f32", budget_bytes: int = 240 * 10 ** 9,
-                chunk: int = WINDOW * SAMPLE_EVERY, seed: int = 7, log=None) -> tuple[list[Solve], float]:
-    """Every case resident in one batch (admission included in the time), advanced `chunk`
-    steps per launch sequence; converged cases leave the batch. `lattices` holds the
-    (config, mask) of each case. Returns the solves and the wall time."""
+                chunk: int = WINDOW * SAMPLE_EVERY, seed: int = 7, log=None,
+                tune_steps: int = 10) -> tuple[list[Solve], float]:
+    """Every case resident in one batch (admission and, with `tune_steps`, the launch autotuner
+    included in the time), advanced `chunk` steps per launch sequence; converged cases leave
+    the batch. `lattices` holds the (config, mask) of each case. Returns the solves and the wall
+    time."""
     t0 = time.perf_counter()
     batch = LatticeBatch(budget_bytes=budget_bytes, sample_every=SAMPLE_EVERY)
     active = {}
     for case, (config, mask) in zip(cases, lattices):
         active[case.index] = (case, batch.add(config, mask, storage, seed, 1e-4))
+    if tune_steps > 0:
+        chosen = autotune.autotune_batch(batch, tune_steps)
+        if log:
+            log(f"batched: launch shapes {chosen}")
     done = []
     while active:
         batch.run(chunk)

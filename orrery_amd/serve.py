@@ -17,6 +17,7 @@ import time
 
 import numpy as np
 
+from . import autotune
 from . import geometry as geo
 from .batch import LatticeBatch
 from .config import Lbm3DConfig, Outlet, Smagorinsky, SolidWall, Sponge, Wall, WallKind, XBoundary
@@ -144,11 +145,12 @@ class Server:
     threads as requests arrive; the device batch advances `chunk` steps at a time."""
 
     def __init__(self, budget_bytes: int = 240 * 10 ** 9, chunk: int = WINDOW * SAMPLE_EVERY, workers: int = 4,
-                 seed: int = 5):
+                 seed: int = 5, tune_steps: int = 10):
         self.budget = budget_bytes
         self.chunk = chunk
         self.workers = workers
         self.seed = seed
+        self.tune_steps = tune_steps  # launch autotuner, once per storage format (0: default shapes)
         self.busy = 0.0  # seconds inside batch.run
 
     def run(self, reqs: list[Request], log=None) -> tuple[list[Served], float]:
@@ -159,6 +161,7 @@ class Server:
         waiting = []    # built, not admitted, in arrival order
         active = {}     # id -> (request, lattice, admitted)
         done = []
+        tuned = set()
         t0 = time.perf_counter()
         try:
             while pending or building or waiting or active:
@@ -187,6 +190,12 @@ class Server:
                     else:
                         time.sleep(0.002)
                     continue
+                storages = {r.storage for r, _, _ in active.values()}
+                if self.tune_steps > 0 and not storages <= tuned:
+                    chosen = autotune.autotune_batch(batch, self.tune_steps)
+                    tuned |= set(chosen)
+                    if log:
+                        log(f"serve: launch shapes {chosen}")
                 t1 = time.perf_counter()
                 batch.run(self.chunk)
                 self.busy += time.perf_counter() - t1

@@ -1,0 +1,74 @@
+"""The autotuner: tuning a lattice mid-run changes none of its later results (the lattice is
+restored bit for bit, and every candidate computes the same values), the fastest candidate is
+the one applied, the JSON cache answers a second lattice of the same shape, and a tuned batch
+gives the histories of an untuned one."""
+import json
+
+import numpy as np
+import pytest
+
+from cases import sphere_channel
+from orrery_amd import Lbm3DSolver, Smagorinsky, SolidWall, autotune
+from orrery_amd.batch import LatticeBatch
+
+pytestmark = pytest.mark.gpu
+
+
+def lattice():
+    c, mask = sphere_channel(70, 21, 19, 7.0, 2.0, 0.05, 60.0)
+    c.smagorinsky = Smagorinsky(True, 0.17)
+    c.solid_wall = SolidWall.interpolated
+    return c, mask
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16s"])
+def test_tuning_mid_run_changes_no_result(storage):
+    c, mask = lattice()
+    plain, tuned = Lbm3DSolver(c, mask, storage), Lbm3DSolver(c, mask, storage)
+    for s in (plain, tuned):
+        s.initialize_perturbed(5, 1e-4)
+        s.step(37)
+    best, tried = autotune.tune(tuned, steps=4)
+    print(f"{storage}: best {best}; {len(tried)} candidates", flush=True)
+    assert tuned.steps() == plain.steps() == 37
+    assert len(tried) == 13 and best.ms_per_step == min(t.ms_per_step for t in tried)
+    assert autotune.current(tuned) == (best.schedule, best.block, best.cells_per_thread)
+    assert np.array_equal(tuned.populations(), plain.populations())
+    for s in (plain, tuned):
+        s.step(50)
+    assert np.array_equal(tuned.populations(), plain.populations())
+    assert tuned.force() == plain.force()
+
+
+def test_cache_answers_the_second_lattice(tmp_path):
+    c, mask = lattice()
+    path = str(tmp_path / "launch.json")
+    a = Lbm3DSolver(c, mask)
+    first = autotune.autotune(a, steps=2, cache=path)
+    data = json.load(open(path))
+    key = autotune.cache_key(a)
+    assert key in data and data[key]["block"] == first.block
+    b = Lbm3DSolver(c, mask)
+    second = autotune.autotune(b, steps=2, cache=path)
+    assert second == first
+    assert autotune.current(b) == (first.schedule, first.block, first.cells_per_thread)
+
+
+def test_tuned_batch_gives_the_untuned_histories():
+    c, mask = lattice()
+    small_c, small_mask = sphere_channel(40, 16, 16, 6.0, 2.0, 0.05, 50.0)
+    runs = []
+    for tune in (False, True):
+        b = LatticeBatch(sample_every=10)
+        lats = [b.add(c, mask, "f32", 3, 1e-4), b.add(small_c, small_mask, "f16s", 4, 1e-4)]
+        b.run(20)
+        if tune:
+            chosen = autotune.autotune_batch(b, steps=2)
+            print(f"batch choices: {chosen}", flush=True)
+            assert set(chosen) == {"f32", "f16s"}
+            assert all(ch.schedule == "serial" for ch in chosen.values())
+        b.run(60)
+        runs.append([lat.history() for lat in lats])
+        b.close()
+    for (t0, f0), (t1, f1) in zip(*runs):
+        assert np.array_equal(t0, t1) and np.array_equal(f0, f1)
