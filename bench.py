@@ -46,7 +46,7 @@ def workload(nx=512, ny=256, nz=256, diameter=64.0, pitch_deg=20.0, reynolds=100
     centre = (2.0 * diameter, 0.5 * (ny - 1) + 0.25, 0.5 * (nz - 1) + 0.25)
     mesh = body_mesh("capsule", diameter).transformed(geometry.rotation_y(math.radians(pitch_deg)), (0.0, 0.0, 0.0),
                                                        centre)
-    return config, place_body(mesh, nx, ny, nz, interpolated=False)
+    return config, place_body(mesh, nx, ny, nz, interpolated=False, device=True)
 
 
 def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare_steps=0, ref=None, tune_steps=0):

@@ -327,6 +327,18 @@ int lbm_signed_distance_band(const lbm_mesh* m, const double origin[3], double s
                              uint8_t* in_band);
 /* MeshTree::winding_number at n points [n][3]. */
 int lbm_winding_numbers(const lbm_mesh* m, const double* points, size_t n, double* out);
+/* lbm_voxelize on the device (no reference counterpart: the reference voxelises on the host):
+ * one thread per cell walks the same tree. The mask equals lbm_voxelize's bit for bit: cells
+ * whose device winding number lies within 1e-9 of the threshold are decided again by the host
+ * evaluation (*rechecked, optional, counts them); elsewhere device and host numbers differ by
+ * a few ulps of atan2 at most (`winding`, optional, holds the numbers that decided). */
+int lbm_voxelize_device(const lbm_mesh* m, const double origin[3], double spacing, uint64_t nx, uint64_t ny,
+                        uint64_t nz, double threshold, uint8_t* solid, double* winding, size_t* rechecked);
+/* lbm_signed_distance_band on the device: bit for bit the host's (additions, products,
+ * divisions, square roots and comparisons in the host's order, all correctly rounded). */
+int lbm_signed_distance_band_device(const lbm_mesh* m, const double origin[3], double spacing, uint64_t nx,
+                                    uint64_t ny, uint64_t nz, const uint8_t* solid, double band_cells,
+                                    double* distance, uint8_t* in_band);
 
 /* ---- the aerodynamic adapter (LbmAeroSolver3D, src/aero/lbm3d/lbm_aero_solver3d.h:45-166) ---- */
 /* A closed body mesh in SI units (body axes, metres) goes in; coefficients, SI force and moment,

@@ -201,6 +201,11 @@ EXPORTS = [
                                                 ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint8),
                                                 ctypes.c_double, c_f64p, P(ctypes.c_uint8)]),
     ("lbm_winding_numbers", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_size_t, c_f64p]),
+    ("lbm_voxelize_device", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64,
+                                           ctypes.c_uint64, ctypes.c_double, P(ctypes.c_uint8), c_f64p, c_sizep]),
+    ("lbm_signed_distance_band_device", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_double, ctypes.c_uint64,
+                                                       ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint8),
+                                                       ctypes.c_double, c_f64p, P(ctypes.c_uint8)]),
     ("lbm_aero3d_last_error", ctypes.c_char_p, []),
     ("lbm_aero3d_default_options", None, [P(CAeroOptions)]),
     ("lbm_default_flow_conditions", None, [P(CFlowConditions)]),

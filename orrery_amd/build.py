@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["lbm_setup.hip", "lbm3d_kernels.hip", "lbm3d_abi.hip", "lbm2d_kernels.hip", "lbm2d_abi.hip",
-           "lbm_batch.hip"]
+           "lbm_batch.hip", "geom_device.hip"]
 # Host-only sources, compiled by the host C++ compiler with the floating-point policy of the
 # reference's geometry and history code (-O2, no contraction, no fast-math) so that their FP64
 # decisions (voxel masks) match it bit for bit.

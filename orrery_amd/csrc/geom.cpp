@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/orrery_lbm.h"
+#include "geom_tree.h"
 
 namespace {
 
@@ -468,6 +469,37 @@ public:
         return std::sqrt(best2);
     }
 
+    void flatten(lbm::geom::FlatTree& out) const {
+        out.scale = scale_;
+        out.beta = beta_;
+        out.nodes.resize(nodes_.size());
+        for (std::size_t k = 0; k < nodes_.size(); ++k) {
+            const Node& n = nodes_[k];
+            lbm::geom::FlatNode& f = out.nodes[k];
+            f = lbm::geom::FlatNode{{n.centre.x, n.centre.y, n.centre.z},
+                                    {n.area_normal.x, n.area_normal.y, n.area_normal.z},
+                                    n.radius,
+                                    {n.box.min.x, n.box.min.y, n.box.min.z},
+                                    {n.box.max.x, n.box.max.y, n.box.max.z},
+                                    n.first,
+                                    n.count,
+                                    n.left,
+                                    n.right,
+                                    n.leaf ? 1U : 0U,
+                                    0U};
+        }
+        out.tri.resize(9 * order_.size());
+        for (std::size_t i = 0; i < order_.size(); ++i) {
+            const Tri& t = m_.t[order_[i]];
+            for (int c = 0; c < 3; ++c) {
+                const V3& v = m_.v[t[c]];
+                out.tri[9 * i + 3 * c + 0] = v.x;
+                out.tri[9 * i + 3 * c + 1] = v.y;
+                out.tri[9 * i + 3 * c + 2] = v.z;
+            }
+        }
+    }
+
 private:
     struct Node {
         Box box;
@@ -603,6 +635,21 @@ bool grid_ok(const double* origin, double spacing, std::uint64_t nx, std::uint64
 struct lbm_mesh {
     Mesh m;
 };
+
+namespace lbm::geom {
+
+void flatten_tree(const lbm_mesh* m, FlatTree& out) {
+    const Tree tree(m->m);
+    tree.flatten(out);
+}
+
+bool check_grid(const double* origin, double spacing, std::uint64_t nx, std::uint64_t ny, std::uint64_t nz) {
+    return grid_ok(origin, spacing, nx, ny, nz);
+}
+
+int fail(const std::string& message) { return geom_fail(message); }
+
+} // namespace lbm::geom
 
 extern "C" {
 

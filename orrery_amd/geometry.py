@@ -206,6 +206,7 @@ class VoxelGrid:
     spec: GridSpec
     solid: np.ndarray  # uint8 [cells]
     winding: np.ndarray  # float64 [cells]
+    rechecked: int | None = None  # device voxelisation: cells decided by the host evaluation
 
 
 @dataclass
@@ -216,20 +217,32 @@ class SignedDistanceField:
     in_band: np.ndarray = field(default=None)  # uint8 [cells]
 
 
-def voxelize(mesh: TriangleMesh, spec: GridSpec, threshold: float = 0.5) -> VoxelGrid:
-    """Solid where the generalised winding number at the cell centre is at least `threshold`."""
+def voxelize(mesh: TriangleMesh, spec: GridSpec, threshold: float = 0.5, device: bool = False) -> VoxelGrid:
+    """Solid where the generalised winding number at the cell centre is at least `threshold`.
+    With `device`, the cells are evaluated on the GPU (the same mask bit for bit; near-threshold
+    cells are decided by the host evaluation, counted in `rechecked`)."""
     lib = _lib()
     solid = np.empty(spec.cells, dtype=np.uint8)
     winding = np.empty(spec.cells)
     o = np.asarray(spec.origin, dtype=np.float64)
+    rechecked = None
     with mesh._handle() as h:
-        _check(lib, lib.lbm_voxelize(h, abi.ptr(o, ctypes.c_double), spec.spacing, spec.nx, spec.ny, spec.nz,
-                                     threshold, abi.ptr(solid, ctypes.c_uint8), abi.ptr(winding, ctypes.c_double)))
-    return VoxelGrid(spec, solid, winding)
+        if device:
+            n = ctypes.c_size_t()
+            _check(lib, lib.lbm_voxelize_device(h, abi.ptr(o, ctypes.c_double), spec.spacing, spec.nx, spec.ny, spec.nz,
+                                                threshold, abi.ptr(solid, ctypes.c_uint8),
+                                                abi.ptr(winding, ctypes.c_double), ctypes.byref(n)))
+            rechecked = n.value
+        else:
+            _check(lib, lib.lbm_voxelize(h, abi.ptr(o, ctypes.c_double), spec.spacing, spec.nx, spec.ny, spec.nz,
+                                         threshold, abi.ptr(solid, ctypes.c_uint8), abi.ptr(winding, ctypes.c_double)))
+    return VoxelGrid(spec, solid, winding, rechecked)
 
 
-def signed_distance_band(mesh: TriangleMesh, voxels: VoxelGrid, band_cells: float = 3.0) -> SignedDistanceField:
-    """Distance to the surface, exact within band_cells * spacing, negative in solid voxels."""
+def signed_distance_band(mesh: TriangleMesh, voxels: VoxelGrid, band_cells: float = 3.0,
+                         device: bool = False) -> SignedDistanceField:
+    """Distance to the surface, exact within band_cells * spacing, negative in solid voxels
+    (with `device`, computed on the GPU, bit for bit the same)."""
     lib = _lib()
     spec = voxels.spec
     if voxels.solid.size != spec.cells:
@@ -238,10 +251,11 @@ def signed_distance_band(mesh: TriangleMesh, voxels: VoxelGrid, band_cells: floa
     band = np.empty(spec.cells, dtype=np.uint8)
     o = np.asarray(spec.origin, dtype=np.float64)
     solid = np.ascontiguousarray(voxels.solid, dtype=np.uint8)
+    fn = lib.lbm_signed_distance_band_device if device else lib.lbm_signed_distance_band
     with mesh._handle() as h:
-        _check(lib, lib.lbm_signed_distance_band(h, abi.ptr(o, ctypes.c_double), spec.spacing, spec.nx, spec.ny,
-                                                 spec.nz, abi.ptr(solid, ctypes.c_uint8), band_cells,
-                                                 abi.ptr(d, ctypes.c_double), abi.ptr(band, ctypes.c_uint8)))
+        _check(lib, fn(h, abi.ptr(o, ctypes.c_double), spec.spacing, spec.nx, spec.ny, spec.nz,
+                       abi.ptr(solid, ctypes.c_uint8), band_cells, abi.ptr(d, ctypes.c_double),
+                       abi.ptr(band, ctypes.c_uint8)))
     return SignedDistanceField(spec, band_cells * spec.spacing, d, band)
 
 

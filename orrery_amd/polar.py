@@ -69,21 +69,22 @@ def wing_cases(n: int = 48, seed: int = 2024, aoa_range=(-4.0, 18.0)) -> list[Wi
 
 
 def place_body(mesh: geo.TriangleMesh, nx: int, ny: int, nz: int, interpolated: bool = True,
-               margin: int = 4) -> SolidMask3D:
+               margin: int = 4, device: bool = False) -> SolidMask3D:
     """Voxelises `mesh` (lattice units: node (i, j, k) sits at (i, j, k)) over its bounding box
     plus `margin` nodes, clipped to the lattice, and places it in a fluid mask; with
-    `interpolated`, the signed distance within three cells becomes the mask's distance."""
+    `interpolated`, the signed distance within three cells becomes the mask's distance. With
+    `device`, the voxels and distances are computed on the GPU (the same values bit for bit)."""
     lo, hi = mesh.bounds()
     dims = (nx, ny, nz)
     first = [max(0, int(math.floor(lo[a])) - margin) for a in range(3)]
     last = [min(dims[a] - 1, int(math.ceil(hi[a])) + margin) for a in range(3)]
     count = [last[a] - first[a] + 1 for a in range(3)]
     spec = geo.GridSpec(tuple(first[a] - 0.5 for a in range(3)), 1.0, *count)
-    vox = geo.voxelize(mesh, spec)
+    vox = geo.voxelize(mesh, spec, device=device)
     mask = fluid_mask_3d(nx, ny, nz)
     geo.place_voxels(mask, vox, first)
     if interpolated:
-        geo.place_distance(mask, geo.signed_distance_band(mesh, vox, 3.0), first)
+        geo.place_distance(mask, geo.signed_distance_band(mesh, vox, 3.0, device=device), first)
     return mask
 
 
