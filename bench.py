@@ -186,6 +186,12 @@ def polar_bench(args):
     same = all((s.steps, s.cd, s.cl) == (by_case[s.case.index].steps, by_case[s.case.index].cd,
                                          by_case[s.case.index].cl) for s in sequential)
     rate = len(batched) / tb * 3600.0
+    from orrery_amd import experiment
+    comparison_table = experiment.compare_by_geometry(polar.records(batched))[0]
+    if args.csv:
+        with open(args.csv, "w", newline="") as f:
+            f.write(polar.comparison_csv(batched))
+        log(f"polar: comparison table written to {args.csv}")
     line = {
         "metric": "batched Cd solves per hour (wing-section sweep, convergence |dCd| < 1e-4 over 500 steps)",
         "value": round(rate, 1), "unit": "solves/h", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
@@ -198,6 +204,8 @@ def polar_bench(args):
         "same_results_as_sequential": same,
         "converged": sum(s.converged for s in batched),
         "lift_slope_per_rad": round(polar.lift_slope(batched), 4),
+        "lift_slope_all_angles_per_rad": round(comparison_table.lift_slope_per_rad, 4)
+        if comparison_table.lift_slope_per_rad is not None else None,
         "glups_batched": round(sum(work.values()) / tb / 1e9, 2),
     }
     print(json.dumps(line), flush=True)
@@ -337,6 +345,7 @@ def main(argv=None):
     ap.add_argument("--sequential-every", type=int, default=1,
                     help="polar: time the one-at-a-time baseline on every k-th case (k > 1 bounds the run)")
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
+    ap.add_argument("--csv", default="", help="polar: write the comparison table (CSV) of the batched solves here")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device with a plain copy kernel (3 launches), the "
                          "known-size reference for the HBM byte counters of a profiler run")

@@ -195,6 +195,33 @@ def run_sequential(cases: list[WingCase], lattices, storage: str = "f32", seed: 
     return done, time.perf_counter() - t0
 
 
+def records(solves: list[Solve], storage: str = "f32") -> list:
+    """The solves as aero experiment records for the comparison tables: a NACA 0012 section of
+    unit chord (the span is periodic), the angle of attack, and the lattice result (Cd, Cl, the
+    steps run, convergence). All cases share the geometry, so they form one table."""
+    from . import experiment as ex
+    out = []
+    for s in solves:
+        e = ex.AeroExperiment(name=f"case {s.case.index:02d} ({storage})")
+        e.geometry = ex.GeometryRef(shape=ex.AeroShape.naca_section, naca="0012",
+                                    dimensions=ex.ShapeDimensions(chord_m=1.0))
+        e.conditions.angle_of_attack_deg = s.case.aoa_deg
+        e.solver = ex.AeroSolverSettings(solver=ex.SolverKind.lbm, reference_area=ex.ReferenceAreaKind.planform,
+                                         reference_length_m=1.0, lbm_cells_per_length=float(s.case.chord))
+        e.result = ex.AeroResultRecord(solver=ex.SolverKind.lbm, cd=s.cd, cl=s.cl, reynolds=s.case.reynolds,
+                                       reference_area_m2=1.0, reference_length_m=1.0, iterations=s.steps,
+                                       converged=s.converged)
+        out.append(e)
+    return out
+
+
+def comparison_csv(solves: list[Solve], storage: str = "f32") -> str:
+    """The sweep's comparison table as CSV (comparison_csv, src/app/aero/aero_comparison.cpp:166-181)."""
+    from . import experiment as ex
+    tables = ex.compare_by_geometry(records(solves, storage))
+    return "".join(ex.comparison_csv(t) for t in tables)
+
+
 def lift_slope(solves: list[Solve], attached=(-4.0, 8.0)) -> float:
     """Least-squares dCl/dalpha per radian over the cases with alpha in `attached` degrees."""
     pts = [(math.radians(s.case.aoa_deg), s.cl) for s in solves if attached[0] <= s.case.aoa_deg <= attached[1]]
@@ -206,4 +233,4 @@ def lift_slope(solves: list[Solve], attached=(-4.0, 8.0)) -> float:
 
 
 __all__ = ["WingCase", "wing_cases", "wing_lattice", "place_body", "run_batched", "run_sequential", "lift_slope",
-           "coefficients", "converged", "Solve"]
+           "coefficients", "converged", "Solve", "records", "comparison_csv"]
