@@ -505,60 +505,94 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
     const R* __restrict__ st = p.stash + (kLocal ? 19U * bs : 0U);
     R* __restrict__ sw = p.stash + (kLocal ? 0U : 19U * bs);
 
-    // Natural pull of direction i at this node (its own address in the A-A scheme).
-    const auto natural = [&](int i, std::uint32_t src) -> R {
-        if constexpr (kLocal) {
-            return T::load(f[i * stride + cell]);
-        } else {
-            return T::load(f[L3::opp(i) * stride + src]);
-        }
-    };
-
-    R v[19];
-    R wall_rho = R(-1);
+    // Every value a rule needs is loaded first, each load depending only on the codes: per
+    // direction the population from f (the natural pull, or the farther node of a short
+    // bounce-back link), another node's stash value (mirrored and outlet sources), the wall
+    // fraction and the outlet history, and the node's own stash slot when a bounce-back or wall
+    // rule reads it. Loads a direction's rule does not need are not issued. The rules below then
+    // compute exactly what they computed with their loads in line.
+    std::uint32_t code[19];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        const std::uint32_t code = p.bcode[i * bs + b];
-        const int kind = static_cast<int>(code & 7U);
-        const std::uint32_t payload = code >> kCodeShift;
+        code[i] = p.bcode[i * bs + b];
+    }
+    bool needs_own = false;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const int kind = static_cast<int>(code[i] & 7U);
+        needs_own = needs_own || kind == kBounce || kind == kWall;
+    }
+    R own[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        own[i] = needs_own ? st[i * bs + b] : R(0);
+    }
+    S fv[19];
+    R sv[19];
+    R q[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const int kind = static_cast<int>(code[i] & 7U);
+        const std::uint32_t payload = code[i] >> kCodeShift;
+        const bool plain = (code[i] & (kCodeMirrorY | kCodeMirrorZ)) == 0U;
+        const bool far = kind == kBounce;  // the farther node of a short link (used when q < 1/2)
+        const bool pull = kind == kNode && plain;
+        if constexpr (kLocal) {
+            // A step: natural pulls are the node's own addresses, direction i or opposite(i)
+            fv[i] = (pull || far) ? f[(far ? L3::opp(i) : i) * stride + cell] : S(0);
+        } else {
+            const std::uint32_t src = far ? static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane)) : payload;
+            fv[i] = (pull || far) ? f[(far ? i : L3::opp(i)) * stride + src] : S(0);
+        }
+        const bool other = (kind == kNode && !plain) || kind == kOutlet;
+        sv[i] = other ? st[mirrored(i, code[i]) * bs + payload] : R(0);
+        q[i] = (far && p.bq != nullptr) ? p.bq[i * bs + b] : R(0.5);
+    }
+    R hist[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        hist[i] = (L3::cx(i) < 0 && p.convective && static_cast<int>(code[i] & 7U) == kOutlet)
+                      ? p.hist[L3::outlet_slot(i) * bs + b]
+                      : R(0);
+    }
+    R wall_rho = 0;
+    if (needs_own) {
+        R s = 0;
+#pragma unroll
+        for (int k = 0; k < 19; ++k) {
+            s += own[k];
+        }
+        wall_rho = T::shifted ? R(1) + s : s;
+    }
+
+    R v[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const int kind = static_cast<int>(code[i] & 7U);
+        const std::uint32_t payload = code[i] >> kCodeShift;
         if (kind == kNode) {
-            if ((code & (kCodeMirrorY | kCodeMirrorZ)) == 0U) {
-                v[i] = natural(i, payload);
-            } else {
-                v[i] = st[mirrored(i, code) * bs + payload];
-            }
+            v[i] = (code[i] & (kCodeMirrorY | kCodeMirrorZ)) == 0U ? T::load(fv[i]) : sv[i];
         } else if (kind == kBounce) {
             // reflected (lbm3d_internal.h:185-197)
-            const int o = L3::opp(i);
-            const R leaving = st[o * bs + b];
-            const R q = p.bq != nullptr ? p.bq[i * bs + b] : R(0.5);
-            if (q == R(0.5)) {
+            const R leaving = own[L3::opp(i)];
+            const R qi = q[i];
+            if (qi == R(0.5)) {
                 v[i] = leaving;
-            } else if (q < R(0.5)) {
-                const R farther = natural(o, static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane)));
-                v[i] = R(2) * q * leaving + (R(1) - R(2) * q) * farther;
+            } else if (qi < R(0.5)) {
+                v[i] = R(2) * qi * leaving + (R(1) - R(2) * qi) * T::load(fv[i]);
             } else {
-                v[i] = (leaving + (R(2) * q - R(1)) * st[i * bs + b]) / (R(2) * q);
+                v[i] = (leaving + (R(2) * qi - R(1)) * own[i]) / (R(2) * qi);
             }
         } else if (kind == kWall) {
-            if (wall_rho < R(0)) {
-                R s = 0;
-#pragma unroll
-                for (int k = 0; k < 19; ++k) {
-                    s += st[k * bs + b];
-                }
-                wall_rho = T::shifted ? R(1) + s : s;
-            }
             const R u = payload == 0U ? p.wall_u[0] : (payload == 1U ? p.wall_u[1] : (payload == 2U ? p.wall_u[2] : p.wall_u[3]));
-            v[i] = st[L3::opp(i) * bs + b] + R(6) * R(L3::w(i)) * wall_rho * R(L3::cx(i)) * u;
+            v[i] = own[L3::opp(i)] + R(6) * R(L3::w(i)) * wall_rho * R(L3::cx(i)) * u;
         } else if (kind == kInlet) {
             v[i] = R(0);
         } else {
-            R up = st[mirrored(i, code) * bs + payload];
+            R up = sv[i];
             if (p.convective) {
-                R* h = p.hist + L3::outlet_slot(i) * bs + b;
-                up = (*h + p.outlet_speed * up) / (R(1) + p.outlet_speed);
-                *h = up;
+                up = (hist[i] + p.outlet_speed * up) / (R(1) + p.outlet_speed);
+                p.hist[L3::outlet_slot(i) * bs + b] = up;
             }
             v[i] = up;
         }

@@ -7,10 +7,7 @@
 // replay it kReplays times, or when the previous call asked for the same parameters.
 //
 // ORRERY_LBM_GRAPHS=0 in the environment turns the replays off for every handle created after
-// it is set (each step is then launched on its own). Profile with it set: a rocprofv3
-// kernel-trace run of the graph-replayed D2Q9 channel ended in a host segmentation fault below
-// lbm2d_step (in the runtime or the profiler's tool library), while the same run without the
-// profiler passes every bitwise check.
+// it is set (each step is then launched on its own), e.g. for tools that trace launches one by one.
 #pragma once
 
 #include <hip/hip_runtime.h>
