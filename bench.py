@@ -15,7 +15,8 @@ same lattice on all host cores. Each storage format (f32, f16s, bf16s) is also r
 arithmetic on the device) at that step.
 
 Other workloads (--workload): polar (the batched wing-section sweep), serve (the
-continuous-batching server), d2q9 (the 400 x 160 cylinder channel).
+continuous-batching server), d2q9 (the 400 x 160 cylinder channel), sphere (the 256 x 128 x 128
+sphere channel).
 """
 from __future__ import annotations
 
@@ -317,6 +318,108 @@ def d2q9_bench(args):
     return 0
 
 
+def sphere_channel():
+    """256 x 128 x 128 channel around a voxelised sphere of diameter 32 cells centred at
+    (64, 64, 64): inlet u 0.05, Re 200, Smagorinsky C 0.17, convective outlet, slip sides,
+    halfway bounce-back."""
+    from orrery_amd import Lbm3DConfig, Smagorinsky, Wall, WallKind, XBoundary, geometry, tau_of_viscosity
+    from orrery_amd.polar import place_body
+    u, d, re = 0.05, 32.0, 200.0
+    config = Lbm3DConfig(nx=256, ny=128, nz=128, tau=tau_of_viscosity(u * d / re), x_boundary=XBoundary.inflow_outflow,
+                         y_low=Wall(WallKind.slip), y_high=Wall(WallKind.slip), z_low=Wall(WallKind.slip),
+                         z_high=Wall(WallKind.slip), inlet_ux=u, smagorinsky=Smagorinsky(True, 0.17))
+    mesh = geometry.sphere_mesh(0.5 * d, geometry.RevolutionResolution(64, 0.05)).transformed(
+        None, (0.0, 0.0, 0.0), (64.0, 64.0, 64.0))
+    return config, place_body(mesh, 256, 128, 128, interpolated=False, device=True), u, d
+
+
+def sphere_bench(args):
+    """The sphere channel: f32 and f64 storage on the device from the same seeded start, a force
+    sample every 10 steps, Cd over the last 1000 steps; GLUPS and the bulk kernel's HBM rate; the
+    oracle on the host for a bounded prefix of the same run, compared with the f64 lattice."""
+    import ctypes
+    import numpy as np
+    from orrery_amd import Lbm3DSolver, abi, autotune, history
+    config, mask, u, d = sphere_channel()
+    steps = args.sphere_steps
+    q = 0.5 * u * u * np.pi * (0.5 * d) ** 2
+    lib = abi.load()
+    out = {}
+    for storage in ("f32", "f64"):
+        s = Lbm3DSolver(config, mask, storage)
+        s.initialize_perturbed(2024, 1e-4)
+        if args.tune_steps > 0:
+            autotune.tune(s, args.tune_steps)
+        log(f"sphere: {storage}, {steps} steps with a force sample every 10")
+        t, cd = [], []
+        t0 = time.perf_counter()
+        for k in range(steps // 10):
+            s.step(10)
+            t.append(float((k + 1) * 10))
+            cd.append(s.force().force[0] / q)
+        wall = time.perf_counter() - t0
+        t, cd = np.array(t), np.array(cd)
+        ms, bulk, bnd = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        for rc in (lib.lbm3d_time_steps(s.handle, args.steps, 0, ctypes.byref(ms), None, None),):
+            if rc:
+                raise RuntimeError(lib.lbm_last_error().decode())
+        step_ms = ms.value / args.steps
+        if lib.lbm3d_time_steps(s.handle, 100, 1, ctypes.byref(ms), ctypes.byref(bulk), ctypes.byref(bnd)):
+            raise RuntimeError(lib.lbm_last_error().decode())
+        counts = [ctypes.c_size_t() for _ in range(4)]
+        lib.lbm3d_counts(s.handle, *[ctypes.byref(c) for c in counts])
+        sb = 4 if storage == "f32" else 8
+        out[storage] = {"cd": float(history.mean_since(t, cd, float(steps - 1000))), "cd_last": float(cd[-1]),
+                        "glups": config.cells / (step_ms * 1e-3) / 1e9, "ms_per_step": step_ms,
+                        "sampled_run_s": wall, "bulk_ms": bulk.value / 100, "boundary_ms": bnd.value / 100,
+                        "bulk_gbps": 2 * 19 * sb * counts[0].value / (bulk.value / 100 * 1e-3) / 1e9,
+                        "bulk_nodes": counts[0].value, "solid_nodes": counts[2].value, "links": counts[3].value}
+        s.close()
+    out["f32"]["cd_rel_error_vs_f64"] = abs(out["f32"]["cd"] / out["f64"]["cd"] - 1.0)
+    cpu = None
+    if args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle import Oracle3D, load
+        threads = os.cpu_count() or 1
+        load().orc_set_threads(threads)
+        n = args.cpu_steps
+        log(f"sphere: the oracle, {n} steps on {threads} threads, against the f64 lattice")
+        o = Oracle3D(config, mask)
+        o.initialize_perturbed(2024, 1e-4)
+        t0 = time.perf_counter()
+        o.step(n)
+        dt = time.perf_counter() - t0
+        g = Lbm3DSolver(config, mask, "f64")
+        g.initialize_perturbed(2024, 1e-4)
+        g.step(n)
+        fo, fg = o.force().force[0], g.force().force[0]
+        mo, mg = o.macroscopic(), g.macroscopic()
+        fluid = np.asarray(mo.solid) == 0
+        dv = max(float(np.abs(getattr(mo, a)[fluid] - getattr(mg, a)[fluid]).max())
+                 for a in ("velocity_x", "velocity_y", "velocity_z"))
+        g.close()
+        cpu = {"value": round(config.cells * n / dt / 1e9, 4), "unit": "GLUPS", "cores": threads, "kind": "port",
+               "sample": f"{n} FP64 steps of the same lattice ({dt:.1f} s) from the same seeded start",
+               "drag_rel_diff_vs_f64_device": abs(fg / fo - 1.0), "velocity_max_abs_diff_vs_f64_device": dv}
+    f32 = out["f32"]
+    line = {
+        "metric": "D3Q19 GLUPS and Cd (sphere channel 256x128x128, Re 200, LES)", "value": round(f32["glups"], 3),
+        "unit": "GLUPS", "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": round(f32["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "D3Q19 256x128x128 channel, voxelised sphere D32 at (64,64,64), u 0.05, Re 200, "
+                               "Smagorinsky C 0.17, convective outlet, slip sides, halfway walls, seeded 1e-4 "
+                               f"perturbation; Cd over steps {steps - 1000}-{steps}", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(f32["bulk_gbps"], 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(f32["bulk_gbps"] * 1e9 / HBM_PEAK, 4), "traffic": None,
+                     "kernel": "d3q19_bulk<StoreF32>"},
+        "storages": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                     for k, v in out.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def serve_bench(args):
     from orrery_amd import serve
     result = serve.serve_bench(args.requests, log=log)
@@ -336,11 +439,12 @@ def main(argv=None):
     ap.add_argument("--compare-steps", type=int, default=2000,
                     help="step at which every storage's force and velocity are compared with the f64 storage's "
                          "(0: no comparison, no f64 run)")
-    ap.add_argument("--workload", choices=["channel", "polar", "serve", "d2q9"], default="channel",
+    ap.add_argument("--workload", choices=["channel", "polar", "serve", "d2q9", "sphere"], default="channel",
                     help="channel: the 512x256x256 lattice (the headline line); polar: the batched wing-section "
                          "sweep, batched against one at a time; serve: the continuous-batching server; d2q9: the "
-                         "400x160 cylinder channel")
+                         "400x160 cylinder channel; sphere: the 256x128x128 sphere channel")
     ap.add_argument("--d2q9-steps", type=int, default=20000, help="d2q9: configured steps (the run is twice as long)")
+    ap.add_argument("--sphere-steps", type=int, default=5000, help="sphere: steps of the sampled runs")
     ap.add_argument("--lattices", type=int, default=48, help="polar: number of wing cases")
     ap.add_argument("--sequential-every", type=int, default=1,
                     help="polar: time the one-at-a-time baseline on every k-th case (k > 1 bounds the run)")
@@ -362,6 +466,8 @@ def main(argv=None):
         return serve_bench(args)
     if args.workload == "d2q9":
         return d2q9_bench(args)
+    if args.workload == "sphere":
+        return sphere_bench(args)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
     config, mask = workload()

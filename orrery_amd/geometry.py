@@ -117,7 +117,8 @@ def _made(call) -> TriangleMesh:
 
 @dataclass
 class RevolutionResolution:
-    """Segments around the axis and the arc-length step along the profile (primitives.h:44-47)."""
+    """Segments around the axis, and the largest angle (radians) of one piece of a profile arc
+    (primitives.h:43-47)."""
 
     around: int = 64
     arc_step: float = 0.05

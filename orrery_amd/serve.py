@@ -80,13 +80,13 @@ def body_mesh(body: str, length: float) -> geo.TriangleMesh:
     """The body, `length` cells long along +x, centred on the origin."""
     L = float(length)
     if body == "sphere":
-        return geo.sphere_mesh(0.5 * L, geo.RevolutionResolution(32, 0.1 * L))
+        return geo.sphere_mesh(0.5 * L, geo.RevolutionResolution(32, 0.1))
     if body == "cone":
         # apex forward: the axis (z, base at 0) turned onto -x
-        m = geo.cone_mesh(0.35 * L, L, geo.RevolutionResolution(32, 0.1 * L))
+        m = geo.cone_mesh(0.35 * L, L, geo.RevolutionResolution(32, 0.1))
         return m.transformed(geo.rotation_y(-0.5 * math.pi), (0.0, 0.0, 0.5 * L))
     if body == "cylinder":
-        m = geo.cylinder_mesh(0.25 * L, L, geo.RevolutionResolution(32, 0.1 * L))
+        m = geo.cylinder_mesh(0.25 * L, L, geo.RevolutionResolution(32, 0.1))
         return m.transformed(geo.rotation_y(0.5 * math.pi), (0.0, 0.0, 0.5 * L))
     if body == "capsule":
         shape = geo.CapsuleShape()
