@@ -241,9 +241,12 @@ def serve_bench(n: int, rate: float = 4.0, log=None, check: int = 3) -> dict:
     queued = np.array([s.queued for s in served])
     first = min(r.arrival for r in reqs)
     last = max(s.finished for s in served)
-    # Cd error of the smallest requests against the device FP64 lattice.
+    # Cd error against the device FP64 lattice run to the same step: the smallest converged
+    # requests (a steady wake; an unsteady one diverges from any rerun in its shedding phase),
+    # else the smallest.
     errors = []
-    for s in sorted(served, key=lambda s: s.request.cells)[:check]:
+    pool = [s for s in served if s.converged] or served
+    for s in sorted(pool, key=lambda s: s.request.cells)[:check]:
         ref = fp64_reference_cd(s.request, s.steps)
         errors.append(abs(s.cd / ref - 1.0) if ref != 0 else abs(s.cd))
         if log:
@@ -259,6 +262,8 @@ def serve_bench(n: int, rate: float = 4.0, log=None, check: int = 3) -> dict:
         "queued_p99_s": round(float(np.percentile(queued, 99)), 3),
         "offered_per_s": rate, "gpu_busy_fraction": round(server.busy / wall, 3), "wall_s": round(wall, 2),
         "converged": sum(s.converged for s in served), "cd_rel_error_vs_fp64_max": max(errors) if errors else None,
+        "cd_checked": [{"request": s.request.id, "storage": s.request.storage, "cells": s.request.cells,
+                        "converged": s.converged} for s in sorted(pool, key=lambda s: s.request.cells)[:check]],
     }
 
 
