@@ -422,7 +422,10 @@ def sphere_bench(args):
 
 def serve_bench(args):
     from orrery_amd import serve
-    result = serve.serve_bench(args.requests, log=log)
+    if args.admission == "all":
+        result = serve.serve_bench(args.requests, log=log, order="arrival", max_active_cells=None)
+    else:
+        result = serve.serve_bench(args.requests, log=log)
     print(json.dumps(result), flush=True)
     return 0
 
@@ -449,6 +452,9 @@ def main(argv=None):
     ap.add_argument("--sequential-every", type=int, default=1,
                     help="polar: time the one-at-a-time baseline on every k-th case (k > 1 bounds the run)")
     ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
+    ap.add_argument("--admission", choices=["shortest", "all"], default="shortest",
+                    help="serve: shortest work first under a cap on resident nodes, or every request that fits, "
+                         "oldest first")
     ap.add_argument("--csv", default="", help="polar: write the comparison table (CSV) of the batched solves here")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device with a plain copy kernel (3 launches), the "

@@ -142,15 +142,27 @@ class Served:
 
 class Server:
     """Runs a list of requests against the wall clock: lattices are prepared on `workers` host
-    threads as requests arrive; the device batch advances `chunk` steps at a time."""
+    threads as requests arrive; the device batch advances `chunk` steps at a time.
+
+    Admission: a built request enters the batch when the HBM budget has room for its lattice and
+    the lattices in the batch hold fewer than `max_active_cells` nodes (one step of that many
+    nodes already saturates HBM, so more resident lattices only share the same bandwidth and
+    finish later); with `order="shortest"` the waiting request with the least work (nodes times
+    its minimum steps) goes first, except that one waiting longer than `max_wait` seconds goes
+    before any other. `max_active_cells=None` and `order="arrival"` admit everything that fits,
+    oldest first."""
 
     def __init__(self, budget_bytes: int = 240 * 10 ** 9, chunk: int = WINDOW * SAMPLE_EVERY, workers: int = 4,
-                 seed: int = 5, tune_steps: int = 10):
+                 seed: int = 5, tune_steps: int = 10, max_active_cells: int | None = 48_000_000,
+                 order: str = "shortest", max_wait: float = 30.0):
         self.budget = budget_bytes
         self.chunk = chunk
         self.workers = workers
         self.seed = seed
         self.tune_steps = tune_steps  # launch autotuner, once per storage format (0: default shapes)
+        self.max_active_cells = max_active_cells
+        self.order = order
+        self.max_wait = max_wait
         self.busy = 0.0  # seconds inside batch.run
 
     def run(self, reqs: list[Request], log=None) -> tuple[list[Served], float]:
@@ -172,10 +184,18 @@ class Server:
                 for rid in [k for k, (_, fut) in building.items() if fut.done()]:
                     r, fut = building.pop(rid)
                     waiting.append((r, fut.result()))
-                waiting.sort(key=lambda item: item[0].arrival)
-                # Admission: every waiting request that fits, oldest first.
+                now = time.perf_counter() - t0
+                if self.order == "shortest":
+                    waiting.sort(key=lambda item: (now - item[0].arrival <= self.max_wait,
+                                                   item[0].cells * item[0].min_steps, item[0].arrival))
+                else:
+                    waiting.sort(key=lambda item: item[0].arrival)
+                active_cells = sum(r.cells for r, _, _ in active.values())
                 for item in list(waiting):
                     r, (config, mask) = item
+                    if self.max_active_cells is not None and active and \
+                            active_cells + r.cells > self.max_active_cells:
+                        break  # keeps the order: nothing passes the request at the head
                     try:
                         lat = batch.add(config, mask, r.storage, self.seed, 1e-4)
                     except LbmError as e:
@@ -183,6 +203,7 @@ class Server:
                             continue
                         raise
                     active[r.id] = (r, lat, time.perf_counter() - t0)
+                    active_cells += r.cells
                     waiting.remove(item)
                 if not active:
                     if pending and not building and not waiting:
@@ -233,9 +254,10 @@ def fp64_reference_cd(req: Request, steps: int, seed: int = 5) -> float:
     return float(drag_coefficient(req, np.array(forces[-WINDOW:])).mean())
 
 
-def serve_bench(n: int, rate: float = 4.0, log=None, check: int = 3) -> dict:
+def serve_bench(n: int, rate: float = 4.0, log=None, check: int = 3, order: str = "shortest",
+                max_active_cells: int | None = 48_000_000) -> dict:
     reqs = requests(n, rate)
-    server = Server()
+    server = Server(order=order, max_active_cells=max_active_cells)
     served, wall = server.run(reqs, log=log)
     lat = np.array([s.latency for s in served])
     queued = np.array([s.queued for s in served])
@@ -261,6 +283,7 @@ def serve_bench(n: int, rate: float = 4.0, log=None, check: int = 3) -> dict:
         "queued_p50_s": round(float(np.percentile(queued, 50)), 3),
         "queued_p99_s": round(float(np.percentile(queued, 99)), 3),
         "offered_per_s": rate, "gpu_busy_fraction": round(server.busy / wall, 3), "wall_s": round(wall, 2),
+        "admission": {"order": order, "max_active_cells": max_active_cells},
         "converged": sum(s.converged for s in served), "cd_rel_error_vs_fp64_max": max(errors) if errors else None,
         "cd_checked": [{"request": s.request.id, "storage": s.request.storage, "cells": s.request.cells,
                         "converged": s.converged} for s in sorted(pool, key=lambda s: s.request.cells)[:check]],
