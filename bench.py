@@ -272,16 +272,19 @@ def d2q9_bench(args):
         series, wall = d2q9_run(s, total, u, d)
         out[storage] = dict(d2q9_summary(series, u, d, steps), mlups_sampled=cells * total / wall / 1e6,
                             wall_s=wall)
-        # Stepping alone: graph replays against one launch pair per step.
-        for graphs in (True, False):
+        # Stepping alone: graph replays against launches one by one, and the one-launch step
+        # against the bulk and boundary kernels launched separately.
+        for name, graphs, schedule in (("mlups_graphs", True, "fused"), ("mlups_eager", False, "fused"),
+                                       ("mlups_two_kernels_graphs", True, "serial")):
             s.set_graphs(graphs)
+            s.set_schedule(schedule)
             s.step(args.steps)  # captures the graph the timed call replays
             s.synchronize()
             t0 = time.perf_counter()
             s.step(args.steps)
             s.synchronize()
-            out[storage]["mlups_graphs" if graphs else "mlups_eager"] = cells * args.steps / (
-                time.perf_counter() - t0) / 1e6
+            out[storage][name] = cells * args.steps / (time.perf_counter() - t0) / 1e6
+        s.set_schedule("fused")
         s.close()
     cpu = None
     if args.cpu_steps > 0:

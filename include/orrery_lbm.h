@@ -202,6 +202,9 @@ int lbm2d_step(lbm2d* h, uint64_t count);
 uint64_t lbm2d_steps(const lbm2d* h);
 /* As lbm3d_set_graphs, for lbm2d_step. */
 int lbm2d_set_graphs(lbm2d* h, int32_t enabled);
+/* LBM_SCHEDULE_FUSED (default): bulk cells and boundary nodes of a step in one launch;
+ * LBM_SCHEDULE_SERIAL: the bulk kernel, then the boundary kernel. Same results bit for bit. */
+int lbm2d_set_schedule(lbm2d* h, int32_t schedule);
 int lbm2d_force(lbm2d* h, double* fx, double* fy, double* moment);
 int lbm2d_mass(lbm2d* h, double* mass);
 int lbm2d_macroscopic(lbm2d* h, double* rho, double* ux, double* uy);

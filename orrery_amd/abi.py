@@ -144,6 +144,7 @@ EXPORTS = [
     ("lbm2d_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
     ("lbm2d_steps", ctypes.c_uint64, [ctypes.c_void_p]),
     ("lbm2d_set_graphs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("lbm2d_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("lbm2d_force", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p]),
     ("lbm2d_mass", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm2d_macroscopic", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p]),

@@ -26,6 +26,9 @@ struct lbm2d {
     hipStream_t stream = nullptr;
     std::uint64_t steps = 0;
     int block = 256;
+    // LBM_SCHEDULE_FUSED: bulk cells and boundary nodes in one launch (default);
+    // LBM_SCHEDULE_SERIAL: the bulk kernel, then the boundary kernel.
+    int schedule = LBM_SCHEDULE_FUSED;
     void* f = nullptr;
     std::uint8_t* cls = nullptr;
     std::uint32_t* bcell = nullptr;
@@ -298,11 +301,24 @@ int lbm2d_step(lbm2d* h, uint64_t count) {
         std::vector<char> key;
         lbm::key_append(key, p);
         lbm::key_append(key, h->block);
+        lbm::key_append(key, h->schedule);
         const hipError_t e = h->graph.run(h->stream, count, h->steps, key, [&](bool local) {
-            return lbm::d2q9_step<T>(p, local, h->block, h->stream);
+            return h->schedule == LBM_SCHEDULE_SERIAL ? lbm::d2q9_step_split<T>(p, local, h->block, h->stream)
+                                                      : lbm::d2q9_step<T>(p, local, h->block, h->stream);
         });
         return e == hipSuccess ? ok() : hip_fail2("step", e);
     });
+}
+
+int lbm2d_set_schedule(lbm2d* h, int32_t schedule) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm2d: null handle");
+    }
+    if (schedule != LBM_SCHEDULE_FUSED && schedule != LBM_SCHEDULE_SERIAL) {
+        return fail(LBM_EINVAL, "lbm2d: the step schedule must be fused or serial");
+    }
+    h->schedule = schedule;
+    return ok();
 }
 
 int lbm2d_set_graphs(lbm2d* h, int32_t enabled) {

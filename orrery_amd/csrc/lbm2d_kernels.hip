@@ -157,11 +157,10 @@ __device__ __forceinline__ void zou_he_west(typename T::R (&f)[9], const Lattice
 }
 
 template <class T, bool kLocal>
-__global__ __launch_bounds__(256) void d2q9_bulk(Lattice2<typename T::R> p) {
+__device__ __forceinline__ void bulk_cell2(const Lattice2<typename T::R>& p, std::uint32_t cell) {
     using S = typename T::S;
     using R = typename T::R;
     S* __restrict__ f = static_cast<S*>(p.f);
-    const std::uint32_t cell = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (cell >= p.cells || p.cls[cell] != 1) {
         return;
     }
@@ -188,10 +187,9 @@ __global__ __launch_bounds__(256) void d2q9_bulk(Lattice2<typename T::R> p) {
 }
 
 template <class T, bool kLocal>
-__global__ __launch_bounds__(256) void d2q9_boundary(Lattice2<typename T::R> p) {
+__device__ __forceinline__ void boundary_node2(const Lattice2<typename T::R>& p, std::uint32_t b) {
     using S = typename T::S;
     using R = typename T::R;
-    const std::uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= p.nb) {
         return;
     }
@@ -278,6 +276,31 @@ __global__ __launch_bounds__(256) void d2q9_boundary(Lattice2<typename T::R> p) 
                 f[i * stride + static_cast<std::uint32_t>(tx + nx * ty)] = packed;
             }
         }
+    }
+}
+
+// The bulk half of a step: one thread per cell (block order renumbered per XCD).
+template <class T, bool kLocal>
+__global__ __launch_bounds__(256) void d2q9_bulk(Lattice2<typename T::R> p) {
+    bulk_cell2<T, kLocal>(p, xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+}
+
+// The boundary half: one thread per boundary node.
+template <class T, bool kLocal>
+__global__ __launch_bounds__(256) void d2q9_boundary(Lattice2<typename T::R> p) {
+    boundary_node2<T, kLocal>(p, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Both halves in one launch: the first `bulk_blocks` blocks take the cells, the others the
+// boundary nodes. Within a step every population address is read and written by one node only
+// and the stash is double-buffered, so the two halves need no order between them; one launch
+// per step is what a lattice small enough to live in cache is bound by.
+template <class T, bool kLocal>
+__global__ __launch_bounds__(256) void d2q9_step_kernel(Lattice2<typename T::R> p, std::uint32_t bulk_blocks) {
+    if (blockIdx.x < bulk_blocks) {
+        bulk_cell2<T, kLocal>(p, xcd_tile(blockIdx.x, bulk_blocks) * blockDim.x + threadIdx.x);
+    } else {
+        boundary_node2<T, kLocal>(p, (blockIdx.x - bulk_blocks) * blockDim.x + threadIdx.x);
     }
 }
 
@@ -538,6 +561,18 @@ inline std::uint32_t blocks2(std::uint64_t n, std::uint32_t block) {
 
 template <class T>
 hipError_t d2q9_step(const Lattice2<typename T::R>& p, bool local, int block, hipStream_t s) {
+    const std::uint32_t bulk = blocks2(p.cells, static_cast<std::uint32_t>(block));
+    const std::uint32_t grid = bulk + blocks2(p.nb, static_cast<std::uint32_t>(block));
+    if (local) {
+        hipLaunchKernelGGL((d2q9_step_kernel<T, true>), dim3(grid), dim3(block), 0, s, p, bulk);
+    } else {
+        hipLaunchKernelGGL((d2q9_step_kernel<T, false>), dim3(grid), dim3(block), 0, s, p, bulk);
+    }
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t d2q9_step_split(const Lattice2<typename T::R>& p, bool local, int block, hipStream_t s) {
     const std::uint32_t grid = blocks2(p.cells, static_cast<std::uint32_t>(block));
     if (local) {
         hipLaunchKernelGGL((d2q9_bulk<T, true>), dim3(grid), dim3(block), 0, s, p);
@@ -609,6 +644,7 @@ hipError_t d2q9_force_sum(const Lattice2<typename T::R>& p, int phase, const std
 
 #define LBM_INSTANTIATE2(T)                                                                                     \
     template hipError_t d2q9_step<T>(const Lattice2<T::R>&, bool, int, hipStream_t);                            \
+    template hipError_t d2q9_step_split<T>(const Lattice2<T::R>&, bool, int, hipStream_t);                      \
     template hipError_t d2q9_initialize<T>(const Lattice2<T::R>&, int, const double*, const double*,            \
                                            std::uint64_t, double, hipStream_t);                                 \
     template hipError_t d2q9_populations<T>(const Lattice2<T::R>&, int, double*, hipStream_t);                  \

@@ -50,6 +50,9 @@ hipError_t d3q19_force_sum(const Lattice3<typename T::R>& p, int phase, const st
 
 template <class T>
 hipError_t d2q9_step(const Lattice2<typename T::R>& p, bool local, int block, hipStream_t s);
+// The same step as two launches (bulk kernel, then boundary kernel).
+template <class T>
+hipError_t d2q9_step_split(const Lattice2<typename T::R>& p, bool local, int block, hipStream_t s);
 template <class T>
 hipError_t d2q9_initialize(const Lattice2<typename T::R>& p, int mode, const double* planes, const double u0[2],
                            std::uint64_t seed, double amplitude, hipStream_t s);

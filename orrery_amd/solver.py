@@ -276,6 +276,10 @@ class Lbm2DSolver:
         """Whether step() replays captured HIP graphs (the default) or launches every step."""
         _check(self._lib, self._lib.lbm2d_set_graphs(self._h, 1 if enabled else 0))
 
+    def set_schedule(self, schedule: str) -> None:
+        """"fused" (bulk and boundary in one launch, the default) or "serial" (two kernels)."""
+        _check(self._lib, self._lib.lbm2d_set_schedule(self._h, abi.SCHEDULE[schedule]))
+
     def initialize(self, initial=None) -> None:
         if initial is None:
             _check(self._lib, self._lib.lbm2d_initialize(self._h, None))
