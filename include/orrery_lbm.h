@@ -208,6 +208,11 @@ int lbm2d_set_schedule(lbm2d* h, int32_t schedule);
 int lbm2d_force(lbm2d* h, double* fx, double* fy, double* moment);
 int lbm2d_mass(lbm2d* h, double* mass);
 int lbm2d_macroscopic(lbm2d* h, double* rho, double* ux, double* uy);
+/* Moments of n listed nodes (cell indices), the values lbm2d_macroscopic reports there; any
+ * output may be NULL. Used for wall-pressure samples without reading the whole field back. */
+int lbm2d_sample(lbm2d* h, const uint32_t* cells, size_t n, double* rho, double* ux, double* uy);
+/* max_speed (lbm2d_common.cpp:168-176): the largest hypot(ux, uy) over the fluid nodes. */
+int lbm2d_max_speed(lbm2d* h, double* speed);
 int lbm2d_populations(lbm2d* h, double* f);
 int lbm2d_links(const lbm2d* h, uint32_t* cell, uint32_t* direction, size_t capacity,
                 size_t* count);
@@ -316,6 +321,11 @@ int lbm_mesh_box(double lx, double ly, double lz, lbm_mesh** out);
 /* naca_wing (naca.cpp:184-257) of Naca4{max_camber, camber_position, thickness}. */
 int lbm_mesh_naca_wing(double max_camber, double camber_position, double thickness, double chord, double span,
                        int32_t n, int32_t span_divisions, lbm_mesh** out);
+/* Naca4::section (src/geom/naca.cpp:155-172): 2 n points (x, z) per unit chord into points[4 n]. */
+int lbm_naca_section(double max_camber, double camber_position, double thickness, int32_t n, double* points);
+/* polygon_mask (src/aero/lbm/solid_mask.cpp:45-73): node (x, y) solid when inside the polygon of
+ * n points (x, z) in lattice units (even-odd rule); solid [nx ny]. */
+int lbm_polygon_mask(uint64_t nx, uint64_t ny, const double* points, size_t n, uint8_t* solid);
 /* v <- R (v - pivot) + translation, R row-major. */
 int lbm_mesh_transform(lbm_mesh* m, const double rotation[9], const double pivot[3], const double translation[3]);
 int lbm_mesh_bounds(const lbm_mesh* m, double lo[3], double hi[3]);
@@ -432,6 +442,74 @@ int lbm_aero3d_lattice(const lbm_aero3d* a, lbm3d** out);
 /* The SI velocity field [cells][3] on the lattice, on request (zero angle of attack and
  * sideslip only). */
 int lbm_aero3d_velocity(lbm_aero3d* a, double* values);
+
+/* ---- the 2D aerodynamic adapter (LbmAeroSolver2D, src/aero/lbm/lbm_aero_solver.h:38-106) ---- */
+/* A closed section (x, z) in SI units (metres, per unit span) goes in; coefficients, SI force and
+ * pitching moment, the force history, the Strouhal number, wall-pressure samples and validity
+ * warnings come out. The section is rotated by the angle of attack so the free stream runs along
+ * lattice x, rasterised by lbm_polygon_mask and placed `upstream` reference lengths behind a
+ * velocity inlet and `downstream` ahead of a convective outlet, in a free-slip channel `height`
+ * lengths high. Results use lbm_aero3d_result (cy = 0; moment about body y).
+ * Errors: lbm_aero2d_last_error(). */
+typedef struct lbm_aero2d_options { /* LbmSolverOptions, lbm_aero_solver.h:38-52 */
+    double cells_per_length, lattice_velocity; /* LatticeResolution{40, 0.05} */
+    double upstream, downstream, height;       /* reference lengths: 8, 16, 16 */
+    int32_t les;                               /* LBM_LES_* */
+    double smagorinsky_constant;               /* 0.1 */
+    double duration, average_window;           /* convective times: 60, 6 */
+    uint64_t steps_per_call, sample_every;     /* 500, 10 */
+    int32_t storage;                           /* LBM_STORE_* of the lattice (no reference counterpart) */
+} lbm_aero2d_options;
+
+typedef struct lbm_aero2d_info { /* what setup() decided */
+    lbm2d_config config;
+    double origin[2];  /* wind-frame position (x, z in m) of lattice node (0, 0) */
+    double centre[2];  /* moment centre, lattice coordinates */
+    double dx, dt, reynolds, lattice_velocity, cells_per_length;
+    uint64_t total_steps;
+    size_t window_samples;
+    double blockage;   /* section height / channel height */
+    size_t solid_nodes;
+} lbm_aero2d_info;
+
+typedef struct lbm_aero2d lbm_aero2d;
+const char* lbm_aero2d_last_error(void);
+void lbm_aero2d_default_options(lbm_aero2d_options* options);
+int lbm_aero2d_create(const lbm_aero2d_options* options, lbm_aero2d** out);
+int lbm_aero2d_destroy(lbm_aero2d* a);
+/* setup (lbm_aero_solver.cpp:39-198) without the device lattice: configuration, mask and
+ * warnings only, on the host. section: n points (x, z), interleaved. */
+int lbm_aero2d_plan(lbm_aero2d* a, const double* section, size_t n, const lbm_reference_values* reference,
+                    const lbm_flow_conditions* conditions);
+/* setup: plan, then the device lattice (lbm2d_create). */
+int lbm_aero2d_setup(lbm_aero2d* a, const double* section, size_t n, const lbm_reference_values* reference,
+                     const lbm_flow_conditions* conditions);
+/* step (lbm_aero_solver.cpp:200-235): *finished = 1 once the run is complete. */
+int lbm_aero2d_step(lbm_aero2d* a, int32_t* finished);
+int lbm_aero2d_get_result(const lbm_aero2d* a, lbm_aero3d_result* out);
+/* CoefficientHistory: sample time (lattice steps), drag and lift coefficients. */
+int lbm_aero2d_history(const lbm_aero2d* a, double* time, double* drag, double* lift, size_t capacity,
+                       size_t* count);
+int lbm_aero2d_residuals(const lbm_aero2d* a, double* residuals, size_t capacity, size_t* count);
+/* Coefficients (drag, lift) of the last sample. */
+int lbm_aero2d_instantaneous(const lbm_aero2d* a, double coefficients[2]);
+/* strouhal() (lbm_aero_solver.cpp:249-263): *found = 0 when the lift shows no oscillation. */
+int lbm_aero2d_strouhal(const lbm_aero2d* a, double* strouhal, int32_t* found);
+/* Wall samples (SurfaceField): points and normals [n][3] in body axes (y = 0); areas (dx, per
+ * metre of span); cp. */
+int lbm_aero2d_surface(const lbm_aero2d* a, double* points, double* normals, double* areas, double* cp,
+                       size_t capacity, size_t* count);
+int lbm_aero2d_warning_count(const lbm_aero2d* a, size_t* count);
+int lbm_aero2d_warning(const lbm_aero2d* a, size_t index, char* code, size_t code_capacity, char* message,
+                       size_t message_capacity);
+int lbm_aero2d_get_info(const lbm_aero2d* a, lbm_aero2d_info* out);
+/* The lattice mask of the plan: solid [cells]. */
+int lbm_aero2d_mask(const lbm_aero2d* a, uint8_t* solid);
+/* The device lattice (borrowed; valid until the next setup or destroy). */
+int lbm_aero2d_lattice(const lbm_aero2d* a, lbm2d** out);
+/* The SI velocity field [cells][3] = (u, 0, w) on the lattice, on request (zero angle of attack
+ * only). */
+int lbm_aero2d_velocity(lbm_aero2d* a, double* values);
 
 /* ---- shared helpers ----------------------------------------------------------------------- */
 /* Seeded perturbation in [-1, 1): SplitMix64 of (seed, cell, component), 53-bit mantissa.

@@ -21,6 +21,7 @@
 #include "orrery/aero/lbm/solid_mask.h"
 #include "orrery/aero/lbm3d/lbm3d.h"
 #include "orrery/aero/lbm3d/solid_mask3d.h"
+#include "orrery/geom/naca.h"
 
 using namespace orrery::aero::lbm;
 
@@ -234,6 +235,35 @@ void ref2d_zou_he_west(double* f, double ux, double uy) {
 void ref2d_circle_mask(std::uint64_t nx, std::uint64_t ny, double cx, double cy, double r, std::uint8_t* solid) {
     const SolidMask2D m = circle_mask(nx, ny, cx, cy, r);
     std::memcpy(solid, m.solid.data(), m.solid.size());
+}
+
+void ref2d_polygon_mask(std::uint64_t nx, std::uint64_t ny, const double* points, std::size_t n, std::uint8_t* solid) {
+    std::vector<orrery::geom::SectionPoint> polygon(n);
+    for (std::size_t k = 0; k < n; ++k) {
+        polygon[k] = orrery::geom::SectionPoint{points[2 * k], points[2 * k + 1]};
+    }
+    const SolidMask2D m = polygon_mask(nx, ny, polygon);
+    std::memcpy(solid, m.solid.data(), m.solid.size());
+}
+
+void ref_naca_section(double max_camber, double camber_position, double thickness, int n, double* out) {
+    orrery::geom::Naca4 a;
+    a.max_camber = max_camber;
+    a.camber_position = camber_position;
+    a.thickness = thickness;
+    const std::vector<orrery::geom::SectionPoint> points = a.section(n);
+    for (std::size_t k = 0; k < points.size(); ++k) {
+        out[2 * k] = points[k].x;
+        out[2 * k + 1] = points[k].z;
+    }
+}
+
+double ref_naca_section_area(double max_camber, double camber_position, double thickness) {
+    orrery::geom::Naca4 a;
+    a.max_camber = max_camber;
+    a.camber_position = camber_position;
+    a.thickness = thickness;
+    return a.section_area();
 }
 
 double ref_lattice_tau(double length, double velocity, double viscosity, double density, double cells_per_length,

@@ -7,14 +7,14 @@ the first solver constructed does, and fails loudly when it is not built.
 from .config import (Lbm2DConfig, Lbm3DConfig, Outlet, Smagorinsky, SolidWall, Sponge, Wall, WallKind,
                      XBoundary, tau_of_viscosity, viscosity_of_tau)
 from .masks import (SolidMask2D, SolidMask3D, box_mask, circle_mask, fluid_mask, fluid_mask_3d,
-                    mask_from_sdf, mask_from_sdf_3d, sphere_mask)
+                    mask_from_sdf, mask_from_sdf_3d, polygon_mask, sphere_mask)
 from .solver import (BoundaryLinks, Force2D, Force3D, Lbm2DSolver, Lbm3DSolver, LbmError, MacroFields2D, boundary_links, classify_nodes, validate,
                      MacroFields3D, device_memory, make_hip_solver, set_memory_limit)
 
 __all__ = [
     "Lbm2DConfig", "Lbm3DConfig", "Outlet", "Smagorinsky", "SolidWall", "Sponge", "Wall", "WallKind",
     "XBoundary", "tau_of_viscosity", "viscosity_of_tau", "SolidMask2D", "SolidMask3D", "box_mask",
-    "circle_mask", "fluid_mask", "fluid_mask_3d", "mask_from_sdf", "mask_from_sdf_3d", "sphere_mask",
+    "circle_mask", "fluid_mask", "fluid_mask_3d", "mask_from_sdf", "mask_from_sdf_3d", "polygon_mask", "sphere_mask",
     "BoundaryLinks", "Force2D", "Force3D", "Lbm2DSolver", "Lbm3DSolver", "LbmError", "MacroFields2D",
     "MacroFields3D", "device_memory", "make_hip_solver", "set_memory_limit", "boundary_links", "classify_nodes",
     "validate",

@@ -97,6 +97,19 @@ class CAeroInfo(ctypes.Structure):
                 ("ramp_steps", D), ("blockage", D), ("surface_area", D), ("solid_nodes", ctypes.c_size_t)]
 
 
+class CAero2DOptions(ctypes.Structure):
+    """lbm_aero2d_options (LbmSolverOptions, lbm_aero_solver.h:38-52)."""
+    _fields_ = [("cells_per_length", D), ("lattice_velocity", D), ("upstream", D), ("downstream", D), ("height", D),
+                ("les", I32), ("smagorinsky_constant", D), ("duration", D), ("average_window", D),
+                ("steps_per_call", U64), ("sample_every", U64), ("storage", I32)]
+
+
+class CAero2DInfo(ctypes.Structure):
+    _fields_ = [("config", CConfig2D), ("origin", D * 2), ("centre", D * 2), ("dx", D), ("dt", D), ("reynolds", D),
+                ("lattice_velocity", D), ("cells_per_length", D), ("total_steps", U64),
+                ("window_samples", ctypes.c_size_t), ("blockage", D), ("solid_nodes", ctypes.c_size_t)]
+
+
 P = ctypes.POINTER
 c_u8p = P(ctypes.c_uint8)
 c_u32p = P(ctypes.c_uint32)
@@ -229,6 +242,33 @@ EXPORTS = [
     ("lbm_aero3d_mask", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint8), c_f64p]),
     ("lbm_aero3d_lattice", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_void_p)]),
     ("lbm_aero3d_velocity", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm_aero2d_last_error", ctypes.c_char_p, []),
+    ("lbm_aero2d_default_options", None, [P(CAero2DOptions)]),
+    ("lbm_aero2d_create", ctypes.c_int, [P(CAero2DOptions), P(ctypes.c_void_p)]),
+    ("lbm_aero2d_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("lbm_aero2d_plan", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_size_t, P(CReferenceValues),
+                                       P(CFlowConditions)]),
+    ("lbm_aero2d_setup", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_size_t, P(CReferenceValues),
+                                        P(CFlowConditions)]),
+    ("lbm_aero2d_step", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int32)]),
+    ("lbm_aero2d_get_result", ctypes.c_int, [ctypes.c_void_p, P(CAeroResult)]),
+    ("lbm_aero2d_history", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p, ctypes.c_size_t, c_sizep]),
+    ("lbm_aero2d_residuals", ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_size_t, c_sizep]),
+    ("lbm_aero2d_instantaneous", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm_aero2d_strouhal", ctypes.c_int, [ctypes.c_void_p, c_f64p, P(ctypes.c_int32)]),
+    ("lbm_aero2d_surface", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p, c_f64p, ctypes.c_size_t,
+                                          c_sizep]),
+    ("lbm_aero2d_warning_count", ctypes.c_int, [ctypes.c_void_p, c_sizep]),
+    ("lbm_aero2d_warning", ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t,
+                                          ctypes.c_char_p, ctypes.c_size_t]),
+    ("lbm_aero2d_get_info", ctypes.c_int, [ctypes.c_void_p, P(CAero2DInfo)]),
+    ("lbm_aero2d_mask", ctypes.c_int, [ctypes.c_void_p, c_u8p]),
+    ("lbm_aero2d_lattice", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_void_p)]),
+    ("lbm_aero2d_velocity", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm2d_sample", ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.c_size_t, c_f64p, c_f64p, c_f64p]),
+    ("lbm2d_max_speed", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
+    ("lbm_naca_section", ctypes.c_int, [D, D, D, I32, c_f64p]),
+    ("lbm_polygon_mask", ctypes.c_int, [U64, U64, c_f64p, ctypes.c_size_t, c_u8p]),
 ]
 
 _lib = None

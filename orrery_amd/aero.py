@@ -1,10 +1,12 @@
-"""The lattice behind Orrery's aerodynamic solver interface (IAeroSolver): LbmAeroSolver3D and its
-inputs and outputs, named as in the reference (src/aero/lbm3d/include/orrery/aero/lbm3d/
-lbm_aero_solver3d.h, src/aero/include/orrery/aero/{aero_solver,flow_conditions,aero_result}.h).
+"""The lattices behind Orrery's aerodynamic solver interface (IAeroSolver): LbmAeroSolver3D and
+LbmAeroSolver2D and their inputs and outputs, named as in the reference
+(src/aero/lbm3d/include/orrery/aero/lbm3d/lbm_aero_solver3d.h,
+src/aero/lbm/include/orrery/aero/lbm/lbm_aero_solver.h,
+src/aero/include/orrery/aero/{aero_solver,flow_conditions,aero_result}.h).
 
-The work is done by the library's adapter (lbm_aero3d_* in include/orrery_lbm.h): the setup
-follows LbmAeroSolver3D::setup operation for operation; the per-call outputs are computed on the
-device.
+The work is done by the library's adapters (lbm_aero3d_* and lbm_aero2d_* in include/orrery_lbm.h):
+the setups follow the reference's setup() operation for operation; the per-call outputs are
+computed on the device.
 """
 from __future__ import annotations
 
@@ -15,9 +17,9 @@ import math
 import numpy as np
 
 from . import abi
-from .config import Lbm3DConfig, Outlet, SolidWall, Wall, WallKind, XBoundary
+from .config import Lbm2DConfig, Lbm3DConfig, Outlet, SolidWall, Wall, WallKind, XBoundary
 from .geometry import TriangleMesh
-from .masks import SolidMask3D
+from .masks import SolidMask2D, SolidMask3D
 
 
 class AeroError(ValueError):
@@ -84,8 +86,12 @@ class ReferenceValues:
 
 @dataclasses.dataclass
 class AeroGeometry:
-    mesh: TriangleMesh
+    """AeroGeometry (aero_solver.h): a closed body mesh for the 3D solver, or a closed section
+    [n, 2] of (x, z) points for the 2D solver, with the reference values."""
+
+    mesh: TriangleMesh | None = None
     reference: ReferenceValues = dataclasses.field(default_factory=ReferenceValues)
+    section: np.ndarray | None = None
 
 
 @dataclasses.dataclass
@@ -220,6 +226,8 @@ class LbmAeroSolver3D:
             pass
 
     def _call(self, fn, geometry: AeroGeometry, conditions: FlowConditions) -> None:
+        if geometry.mesh is None:
+            raise AeroError("lbm3d: invalid mesh: mesh has no vertices or no triangles")
         handle = geometry.mesh._handle()
         with handle as m:
             r = geometry.reference.to_c()
@@ -360,5 +368,226 @@ class LbmAeroSolver3D:
         return out.reshape(c.nz, c.ny, c.nx, 3)
 
 
+# ---- the 2D adapter ------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class LbmSolverOptions:
+    """LbmSolverOptions (lbm_aero_solver.h:38-52); `storage` picks the lattice's population format
+    (no reference counterpart)."""
+
+    cells_per_length: float = 40.0
+    lattice_velocity: float = 0.05
+    upstream: float = 8.0
+    downstream: float = 16.0
+    height: float = 16.0
+    les: int = LesMode.automatic
+    smagorinsky_constant: float = 0.1
+    duration: float = 60.0
+    average_window: float = 6.0
+    steps_per_call: int = 500
+    sample_every: int = 10
+    storage: str = "f32"
+
+    def to_c(self) -> abi.CAero2DOptions:
+        return abi.CAero2DOptions(self.cells_per_length, self.lattice_velocity, self.upstream, self.downstream,
+                                  self.height, int(self.les), self.smagorinsky_constant, self.duration,
+                                  self.average_window, self.steps_per_call, self.sample_every,
+                                  abi.STORAGE[self.storage])
+
+
+@dataclasses.dataclass
+class LatticePlan2D:
+    """What the 2D setup decided: lattice configuration, placement and step counts."""
+
+    config: Lbm2DConfig
+    origin: tuple
+    centre: tuple
+    dx: float
+    dt: float
+    reynolds: float
+    lattice_velocity: float
+    cells_per_length: float
+    total_steps: int
+    window_samples: int
+    blockage: float
+    solid_nodes: int
+
+
+def _config2_from_c(c: abi.CConfig2D) -> Lbm2DConfig:
+    from .config import Smagorinsky
+    wall = lambda w: Wall(WallKind(w.kind), w.velocity_x)  # noqa: E731
+    return Lbm2DConfig(nx=c.nx, ny=c.ny, tau=c.tau, x_boundary=XBoundary(c.x_boundary), outlet=Outlet(c.outlet),
+                       bottom=wall(c.bottom), top=wall(c.top), inlet_ux=c.inlet_ux, inlet_uy=c.inlet_uy,
+                       body_force_x=c.body_force_x, body_force_y=c.body_force_y,
+                       smagorinsky=Smagorinsky(bool(c.smagorinsky_enabled), c.smagorinsky_constant))
+
+
+class LbmAeroSolver2D:
+    """IAeroSolver over the D2Q9 lattice (lbm_aero_solver.h:54-106): a closed section in the x-z
+    plane, per unit span."""
+
+    kind = "lattice_boltzmann"
+
+    def __init__(self, options: LbmSolverOptions | None = None):
+        self.options = options or LbmSolverOptions()
+        self._lib = abi.load()
+        h = ctypes.c_void_p()
+        o = self.options.to_c()
+        self._check(self._lib.lbm_aero2d_create(ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self._result = AeroResult()
+        self._set_up = False
+
+    def _check(self, rc: int) -> None:
+        if rc != abi.LBM_OK:
+            raise AeroError(self._lib.lbm_aero2d_last_error().decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.lbm_aero2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, fn, geometry: AeroGeometry, conditions: FlowConditions) -> None:
+        section = np.zeros((0, 2)) if geometry.section is None else geometry.section
+        pts = np.ascontiguousarray(np.asarray(section, dtype=np.float64).reshape(-1, 2))
+        r = geometry.reference.to_c()
+        c = conditions.to_c()
+        self._check(fn(self._h, abi.ptr(pts, ctypes.c_double), len(pts), ctypes.byref(r), ctypes.byref(c)))
+        self.geometry = geometry
+        self.conditions = conditions
+
+    def plan(self, geometry: AeroGeometry, conditions: FlowConditions) -> LatticePlan2D:
+        """setup() without the device lattice: the lattice, mask and warnings, host only."""
+        self._set_up = False
+        self._call(self._lib.lbm_aero2d_plan, geometry, conditions)
+        self._result = AeroResult(warnings=self._warnings())
+        return self.info()
+
+    def setup(self, geometry: AeroGeometry, conditions: FlowConditions) -> None:
+        self._set_up = False
+        self._call(self._lib.lbm_aero2d_setup, geometry, conditions)
+        self._set_up = True
+        self._result = AeroResult(warnings=self._warnings())
+
+    def step(self) -> bool:
+        if not self._set_up:
+            raise RuntimeError("lbm: step() called before setup()")
+        done = ctypes.c_int32()
+        self._check(self._lib.lbm_aero2d_step(self._h, ctypes.byref(done)))
+        self._result = self._read_result()
+        return bool(done.value)
+
+    def run(self, max_calls: int = 10 ** 7) -> AeroResult:
+        for _ in range(max_calls):
+            if self.step():
+                return self._result
+        raise RuntimeError("lbm: run did not finish")
+
+    def result(self) -> AeroResult:
+        return self._result
+
+    def _warnings(self) -> list:
+        n = ctypes.c_size_t()
+        self._check(self._lib.lbm_aero2d_warning_count(self._h, ctypes.byref(n)))
+        out = []
+        for k in range(n.value):
+            code = ctypes.create_string_buffer(64)
+            msg = ctypes.create_string_buffer(1024)
+            self._check(self._lib.lbm_aero2d_warning(self._h, k, code, 64, msg, 1024))
+            out.append(ValidityWarning(code.value.decode(), msg.value.decode()))
+        return out
+
+    def _read_result(self) -> AeroResult:
+        r = abi.CAeroResult()
+        self._check(self._lib.lbm_aero2d_get_result(self._h, ctypes.byref(r)))
+        n = ctypes.c_size_t()
+        self._check(self._lib.lbm_aero2d_surface(self._h, None, None, None, None, 0, ctypes.byref(n)))
+        pts = np.empty((n.value, 3))
+        nrm = np.empty((n.value, 3))
+        areas = np.empty(n.value)
+        cp = np.empty(n.value)
+        self._check(self._lib.lbm_aero2d_surface(self._h, abi.ptr(pts, ctypes.c_double), abi.ptr(nrm, ctypes.c_double),
+                                                 abi.ptr(areas, ctypes.c_double), abi.ptr(cp, ctypes.c_double),
+                                                 n.value, ctypes.byref(n)))
+        self._check(self._lib.lbm_aero2d_residuals(self._h, None, 0, ctypes.byref(n)))
+        res = np.empty(n.value)
+        self._check(self._lib.lbm_aero2d_residuals(self._h, abi.ptr(res, ctypes.c_double), n.value,
+                                                   ctypes.byref(n)))
+        return AeroResult(cd=r.cd, cl=r.cl, cy=r.cy, cm=r.cm, force=np.array(r.force[:]), moment=np.array(r.moment[:]),
+                          surface=SurfaceField(pts, nrm, areas, cp), residuals=res.tolist(), iterations=r.iterations,
+                          converged=bool(r.converged), warnings=self._warnings())
+
+    def history(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(time, drag, lift) of the samples: CoefficientHistory."""
+        n = ctypes.c_size_t()
+        self._check(self._lib.lbm_aero2d_history(self._h, None, None, None, 0, ctypes.byref(n)))
+        arrays = [np.empty(n.value) for _ in range(3)]
+        self._check(self._lib.lbm_aero2d_history(self._h, *[abi.ptr(a, ctypes.c_double) for a in arrays], n.value,
+                                                 ctypes.byref(n)))
+        return tuple(arrays)
+
+    def moving_average(self) -> tuple[float, float]:
+        """CoefficientHistory::moving_average (force_history.cpp:20-34)."""
+        t, d, l = self.history()
+        n = min(self.info().window_samples, len(t))
+        if n == 0:
+            return 0.0, 0.0
+        sd = 0.0
+        sl = 0.0
+        for k in range(len(t) - n, len(t)):
+            sd += d[k]
+            sl += l[k]
+        return sd / n, sl / n
+
+    def instantaneous(self) -> tuple[float, float]:
+        c = (ctypes.c_double * 2)()
+        self._check(self._lib.lbm_aero2d_instantaneous(self._h, c))
+        return tuple(c[:])
+
+    def strouhal(self) -> float | None:
+        """Shedding frequency f L / U of the lift over the averaging window, None when the lift
+        shows no oscillation (lbm_aero_solver.cpp:249-263)."""
+        st = ctypes.c_double()
+        found = ctypes.c_int32()
+        self._check(self._lib.lbm_aero2d_strouhal(self._h, ctypes.byref(st), ctypes.byref(found)))
+        return st.value if found.value else None
+
+    def info(self) -> LatticePlan2D:
+        i = abi.CAero2DInfo()
+        self._check(self._lib.lbm_aero2d_get_info(self._h, ctypes.byref(i)))
+        return LatticePlan2D(config=_config2_from_c(i.config), origin=tuple(i.origin[:]), centre=tuple(i.centre[:]),
+                             dx=i.dx, dt=i.dt, reynolds=i.reynolds, lattice_velocity=i.lattice_velocity,
+                             cells_per_length=i.cells_per_length, total_steps=i.total_steps,
+                             window_samples=i.window_samples, blockage=i.blockage, solid_nodes=i.solid_nodes)
+
+    def mask(self) -> SolidMask2D:
+        c = self.info().config
+        solid = np.empty(c.cells, dtype=np.uint8)
+        self._check(self._lib.lbm_aero2d_mask(self._h, abi.ptr(solid, ctypes.c_uint8)))
+        return SolidMask2D(c.nx, c.ny, solid)
+
+    def total_steps(self) -> int:
+        return self.info().total_steps
+
+    def lattice_handle(self) -> ctypes.c_void_p:
+        h = ctypes.c_void_p()
+        self._check(self._lib.lbm_aero2d_lattice(self._h, ctypes.byref(h)))
+        return h
+
+    def velocity_field(self) -> np.ndarray:
+        """SI velocity on the lattice [ny, nx, 3] = (u, 0, w) (zero angle of attack only)."""
+        c = self.info().config
+        out = np.empty((c.cells, 3))
+        self._check(self._lib.lbm_aero2d_velocity(self._h, abi.ptr(out, ctypes.c_double)))
+        return out.reshape(c.ny, c.nx, 3)
+
+
 __all__ = ["FlowConditions", "ReferenceValues", "AeroGeometry", "LbmSolver3DOptions", "LbmAeroSolver3D", "AeroResult",
-           "SurfaceField", "ValidityWarning", "LesMode", "LatticePlan", "AeroError"]
+           "SurfaceField", "ValidityWarning", "LesMode", "LatticePlan", "AeroError", "LbmSolverOptions",
+           "LbmAeroSolver2D", "LatticePlan2D"]

@@ -18,7 +18,7 @@ SOURCES = ["lbm_setup.hip", "lbm3d_kernels.hip", "lbm3d_abi.hip", "lbm2d_kernels
 # Host-only sources, compiled by the host C++ compiler with the floating-point policy of the
 # reference's geometry and history code (-O2, no contraction, no fast-math) so that their FP64
 # decisions (voxel masks) match it bit for bit.
-HOST_SOURCES = ["geom.cpp", "lbm_history.cpp", "aero3d.cpp"]
+HOST_SOURCES = ["geom.cpp", "lbm_history.cpp", "aero3d.cpp", "aero2d.cpp"]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-pthread"]
 OUTPUT = os.path.join(HERE, "liborrery_lbm.so")

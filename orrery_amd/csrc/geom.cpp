@@ -226,7 +226,9 @@ struct Naca {
     }
 };
 
-void naca_wing(const Naca& a, double chord, double span, int n, int span_divisions, Mesh& m) {
+// Naca4::section (naca.cpp:155-172): 2 n points (x, z) per unit chord, counter-clockwise from
+// the trailing edge over the upper surface, cosine-spaced stations.
+std::vector<PP> naca_section(const Naca& a, int n) {
     const auto count = static_cast<std::size_t>(n);
     std::vector<PP> section(2 * count); // (x, z) in r, z
     section[0] = {1.0, 0.0};
@@ -239,6 +241,12 @@ void naca_wing(const Naca& a, double chord, double span, int n, int span_divisio
         section[count - s] = {x - yt * std::sin(theta), yc + yt * std::cos(theta)};
         section[count + s] = {x + yt * std::sin(theta), yc - yt * std::cos(theta)};
     }
+    return section;
+}
+
+void naca_wing(const Naca& a, double chord, double span, int n, int span_divisions, Mesh& m) {
+    const auto count = static_cast<std::size_t>(n);
+    const std::vector<PP> section = naca_section(a, n);
     const std::size_t ring = section.size();
     const auto strips = static_cast<std::size_t>(span_divisions);
     m.v.clear();
@@ -818,6 +826,58 @@ int lbm_mesh_naca_wing(double max_camber, double camber_position, double thickne
     }
     naca_wing(Naca{max_camber, camber_position, thickness}, chord, span, n, span_divisions, h->m);
     *out = h;
+    return LBM_OK;
+}
+
+// Naca4::section (naca.cpp:155-172) of Naca4{max_camber, camber_position, thickness}.
+int lbm_naca_section(double max_camber, double camber_position, double thickness, int32_t n, double* points) {
+    if (points == nullptr) {
+        return geom_fail("naca_section: null output");
+    }
+    if (n < 3 || n > 1000000) {
+        return geom_fail("Naca4::section: n must be in [3, 1000000]");
+    }
+    const std::vector<PP> section = naca_section(Naca{max_camber, camber_position, thickness}, n);
+    for (std::size_t k = 0; k < section.size(); ++k) {
+        points[2 * k] = section[k].r;
+        points[2 * k + 1] = section[k].z;
+    }
+    return LBM_OK;
+}
+
+// polygon_mask (src/aero/lbm/solid_mask.cpp:45-73): even-odd crossings of the row through each
+// node, node (x, y) solid when inside the polygon (x, z) given in lattice units.
+int lbm_polygon_mask(uint64_t nx, uint64_t ny, const double* points, size_t n, uint8_t* solid) {
+    if (solid == nullptr || (n > 0 && points == nullptr)) {
+        return geom_fail("polygon_mask: null argument");
+    }
+    if (nx == 0 || ny == 0 || nx > (std::uint64_t{1} << 27U) / ny) {
+        return geom_fail("solid mask: dimensions must be at least 1 and the node count at most geom::max_grid_cells");
+    }
+    std::fill(solid, solid + nx * ny, std::uint8_t{0});
+    if (n < 3) {
+        return LBM_OK;
+    }
+    for (std::uint64_t y = 0; y < ny; ++y) {
+        const auto py = static_cast<double>(y);
+        for (std::uint64_t x = 0; x < nx; ++x) {
+            const auto px = static_cast<double>(x);
+            bool inside = false;
+            for (std::size_t a = 0, b = n - 1; a < n; b = a++) {
+                const double ax = points[2 * a], az = points[2 * a + 1];
+                const double bx = points[2 * b], bz = points[2 * b + 1];
+                if ((az > py) != (bz > py)) {
+                    const double crossing = ax + (py - az) * (bx - ax) / (bz - az);
+                    if (px < crossing) {
+                        inside = !inside;
+                    }
+                }
+            }
+            if (inside) {
+                solid[x + nx * y] = 1;
+            }
+        }
+    }
     return LBM_OK;
 }
 

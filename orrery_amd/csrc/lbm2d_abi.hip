@@ -2,6 +2,7 @@
 // make_gpu_solver for Lbm2DConfig (src/aero/lbm/lbm2d_gpu.cpp:68-378).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -380,6 +381,83 @@ int lbm2d_mass(lbm2d* h, double* mass) {
         e = hipStreamSynchronize(h->stream);
     }
     return e == hipSuccess ? ok() : hip_fail2("mass readback", e);
+}
+
+int lbm2d_sample(lbm2d* h, const uint32_t* cells, size_t n, double* rho, double* ux, double* uy) {
+    if (h == nullptr || (n > 0 && cells == nullptr)) {
+        return fail(LBM_EINVAL, "lbm2d: null argument");
+    }
+    if (n == 0) {
+        return ok();
+    }
+    if (n > std::numeric_limits<std::int32_t>::max()) {
+        return fail(LBM_EINVAL, "lbm2d: too many sample nodes");
+    }
+    // Boundary slot of each node (the boundary list is in cell order), or -1.
+    const std::vector<std::uint32_t>& bcell = h->tables.bcell;
+    std::vector<std::int32_t> slots(n);
+    for (std::size_t k = 0; k < n; ++k) {
+        if (cells[k] >= h->cells) {
+            return fail(LBM_EINVAL, "lbm2d: sample node outside the lattice");
+        }
+        const auto it = std::lower_bound(bcell.begin(), bcell.end(), cells[k]);
+        slots[k] = it != bcell.end() && *it == cells[k] ? static_cast<std::int32_t>(it - bcell.begin()) : -1;
+    }
+    char* buffer = nullptr;
+    const std::size_t bytes = n * (4U + 4U + 3U * sizeof(double));
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&buffer), bytes);
+    if (e != hipSuccess) {
+        return fail(LBM_ENOMEM, std::string("lbm2d gpu: sample buffer: ") + hipGetErrorString(e));
+    }
+    auto* dcells = reinterpret_cast<std::uint32_t*>(buffer);
+    auto* dslots = reinterpret_cast<std::int32_t*>(buffer + 4U * n);
+    auto* dout = reinterpret_cast<double*>(buffer + 8U * n);
+    e = hipMemcpyAsync(dcells, cells, 4U * n, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        e = hipMemcpyAsync(dslots, slots.data(), 4U * n, hipMemcpyHostToDevice, h->stream);
+    }
+    int rc = e == hipSuccess ? LBM_OK : hip_fail2("sample upload", e);
+    if (rc == LBM_OK) {
+        const int phase = static_cast<int>(h->steps & 1U);
+        const double force[2] = {h->cfg.body_force_x, h->cfg.body_force_y};
+        rc = with_store2(*h, [&]<class T>() -> int {
+            const hipError_t err = lbm::d2q9_sample<T>(descriptor2<T>(*h), phase, force, dcells, dslots,
+                                                       static_cast<std::uint32_t>(n), dout, h->stream);
+            return err == hipSuccess ? ok() : hip_fail2("sample", err);
+        });
+    }
+    double* outs[3] = {rho, ux, uy};
+    for (int k = 0; k < 3 && rc == LBM_OK; ++k) {
+        if (outs[k] != nullptr) {
+            e = hipMemcpyAsync(outs[k], dout + k * n, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+            if (e != hipSuccess) {
+                rc = hip_fail2("sample readback", e);
+            }
+        }
+    }
+    const int sy = sync2(*h, "sample readback");
+    (void)hipFree(buffer);
+    return rc != LBM_OK ? rc : sy;
+}
+
+int lbm2d_max_speed(lbm2d* h, double* speed) {
+    if (h == nullptr || speed == nullptr) {
+        return fail(LBM_EINVAL, "lbm2d: null argument");
+    }
+    const int phase = static_cast<int>(h->steps & 1U);
+    const double force[2] = {h->cfg.body_force_x, h->cfg.body_force_y};
+    const int rc = with_store2(*h, [&]<class T>() -> int {
+        const hipError_t e = lbm::d2q9_max_speed<T>(descriptor2<T>(*h), phase, force, h->partial, h->result, h->stream);
+        return e == hipSuccess ? ok() : hip_fail2("max speed", e);
+    });
+    if (rc != LBM_OK) {
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(speed, h->result, sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) {
+        e = hipStreamSynchronize(h->stream);
+    }
+    return e == hipSuccess ? ok() : hip_fail2("max speed readback", e);
 }
 
 int lbm2d_macroscopic(lbm2d* h, double* rho, double* ux, double* uy) {

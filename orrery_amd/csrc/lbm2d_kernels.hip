@@ -10,6 +10,7 @@
 namespace lbm {
 
 template <int K> __global__ void reduce_partials(const double* partial, std::uint32_t n, double* out);
+__global__ void reduce_max(const double* partial, std::uint32_t n, double* out);
 
 namespace {
 
@@ -465,6 +466,79 @@ __global__ __launch_bounds__(256) void d2q9_moments_boundary(Lattice2<typename T
     out[2U * static_cast<std::size_t>(p.cells) + cell] = uy;
 }
 
+// Moments of listed nodes (the values macroscopic() reports there): a node with a boundary
+// slot reads its stash entries, a bulk node its lattice entries, a solid node is at rest.
+template <class T>
+__global__ __launch_bounds__(256) void d2q9_sample(Lattice2<typename T::R> p, int phase, double fx, double fy,
+                                                   const std::uint32_t* cells, const std::int32_t* slots,
+                                                   std::uint32_t n, double* out) {
+    const std::uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) {
+        return;
+    }
+    const std::uint32_t cell = cells[k];
+    const std::int32_t slot = slots[k];
+    double rho = 1.0, ux = 0.0, uy = 0.0;
+    double v[9];
+    if (slot >= 0) {
+        const auto* st = p.stash + (phase ? 9U * p.bstride : 0U);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            v[i] = static_cast<double>(st[i * p.bstride + static_cast<std::uint32_t>(slot)]);
+        }
+        moments2<T>(v, fx, fy, rho, ux, uy);
+    } else if (cell < p.cells && p.cls[cell] == 1) {
+        const auto* f = static_cast<const typename T::S*>(p.f);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            v[i] = static_cast<double>(bulk_value2<T>(p, f, i, cell, phase));
+        }
+        moments2<T>(v, fx, fy, rho, ux, uy);
+    }
+    out[k] = rho;
+    out[n + k] = ux;
+    out[2U * n + k] = uy;
+}
+
+// max_speed (lbm2d_common.cpp:168-176): hypot(u, v) of the moments of each fluid node.
+template <class T>
+__global__ __launch_bounds__(256) void d2q9_speed_cells(Lattice2<typename T::R> p, int phase, double fx, double fy,
+                                                        double* partial) {
+    const std::uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+    double speed = 0.0;
+    if (cell < p.cells && p.cls[cell] == 1) {
+        const auto* f = static_cast<const typename T::S*>(p.f);
+        double v[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            v[i] = static_cast<double>(bulk_value2<T>(p, f, i, cell, phase));
+        }
+        double rho, ux, uy;
+        moments2<T>(v, fx, fy, rho, ux, uy);
+        speed = hypot(ux, uy);
+    }
+    block_max(speed, partial);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void d2q9_speed_boundary(Lattice2<typename T::R> p, int phase, double fx, double fy,
+                                                           double* partial) {
+    const std::uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    double speed = 0.0;
+    if (b < p.nb) {
+        const auto* st = p.stash + (phase ? 9U * p.bstride : 0U);
+        double v[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            v[i] = static_cast<double>(st[i * p.bstride + b]);
+        }
+        double rho, ux, uy;
+        moments2<T>(v, fx, fy, rho, ux, uy);
+        speed = hypot(ux, uy);
+    }
+    block_max(speed, partial);
+}
+
 template <int K>
 __device__ __forceinline__ void block_sum2(double (&v)[K], double* partial) {
     __shared__ double lds[16][K];
@@ -623,6 +697,33 @@ hipError_t d2q9_moments(const Lattice2<typename T::R>& p, int phase, const doubl
 }
 
 template <class T>
+hipError_t d2q9_sample(const Lattice2<typename T::R>& p, int phase, const double force[2], const std::uint32_t* cells,
+                       const std::int32_t* slots, std::uint32_t n, double* out, hipStream_t s) {
+    if (n == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL((d2q9_sample<T>), dim3(blocks2(n, 256)), dim3(256), 0, s, p, phase, force[0], force[1], cells,
+                       slots, n, out);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t d2q9_max_speed(const Lattice2<typename T::R>& p, int phase, const double force[2], double* partial,
+                          double* result, hipStream_t s) {
+    const std::uint32_t nc = blocks2(p.cells, 256);
+    hipLaunchKernelGGL((d2q9_speed_cells<T>), dim3(nc), dim3(256), 0, s, p, phase, force[0], force[1], partial);
+    std::uint32_t n = nc;
+    if (p.nb > 0) {
+        const std::uint32_t nb = blocks2(p.nb, 256);
+        hipLaunchKernelGGL((d2q9_speed_boundary<T>), dim3(nb), dim3(256), 0, s, p, phase, force[0], force[1],
+                           partial + nc);
+        n += nb;
+    }
+    hipLaunchKernelGGL(reduce_max, dim3(1), dim3(256), 0, s, partial, n, result);
+    return hipGetLastError();
+}
+
+template <class T>
 hipError_t d2q9_mass(const Lattice2<typename T::R>& p, int phase, double* partial, double* out, hipStream_t s) {
     const std::uint32_t gc = blocks2(p.cells, 256);
     const std::uint32_t gb = blocks2(p.nb > 0 ? p.nb : 1, 256);
@@ -649,6 +750,10 @@ hipError_t d2q9_force_sum(const Lattice2<typename T::R>& p, int phase, const std
                                            std::uint64_t, double, hipStream_t);                                 \
     template hipError_t d2q9_populations<T>(const Lattice2<T::R>&, int, double*, hipStream_t);                  \
     template hipError_t d2q9_moments<T>(const Lattice2<T::R>&, int, const double*, double*, hipStream_t);       \
+    template hipError_t d2q9_sample<T>(const Lattice2<T::R>&, int, const double*, const std::uint32_t*,         \
+                                       const std::int32_t*, std::uint32_t, double*, hipStream_t);               \
+    template hipError_t d2q9_max_speed<T>(const Lattice2<T::R>&, int, const double*, double*, double*,          \
+                                          hipStream_t);                                                         \
     template hipError_t d2q9_mass<T>(const Lattice2<T::R>&, int, double*, double*, hipStream_t);                \
     template hipError_t d2q9_force_sum<T>(const Lattice2<T::R>&, int, const std::uint32_t*, const std::uint8_t*, \
                                           const std::uint32_t*, std::uint32_t, double*, double*, hipStream_t);

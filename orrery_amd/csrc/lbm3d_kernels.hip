@@ -431,29 +431,6 @@ __global__ __launch_bounds__(256) void d3q19_sample(Lattice3<typename T::R> p, i
     out[3U * n + k] = uz;
 }
 
-// Largest |u| of one block (exact: a maximum does not depend on the order it is taken in).
-__device__ __forceinline__ void block_max(double v, double* partial) {
-    __shared__ double lds[16];
-#pragma unroll
-    for (int offset = 32; offset > 0; offset >>= 1) {
-        v = fmax(v, __shfl_down(v, offset, 64));
-    }
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        lds[wave] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = 0.0;
-        const int waves = static_cast<int>(blockDim.x >> 6);
-        for (int w = 0; w < waves; ++w) {
-            m = fmax(m, lds[w]);
-        }
-        partial[blockIdx.x] = m;
-    }
-}
-
 // max_speed (lbm3d_common.cpp:200-211): sqrt(u^2 + v^2 + w^2) of the moments of each fluid node.
 template <class T>
 __global__ __launch_bounds__(256) void d3q19_speed_cells(Lattice3<typename T::R> p, int phase, double fx, double fy,
@@ -493,25 +470,6 @@ __global__ __launch_bounds__(256) void d3q19_speed_boundary(Lattice3<typename T:
         speed = sqrt(ux * ux + uy * uy + uz * uz);
     }
     block_max(speed, partial);
-}
-
-__global__ __launch_bounds__(256) void reduce_max(const double* partial, std::uint32_t n, double* out) {
-    __shared__ double lds[256];
-    double m = 0.0;
-    for (std::uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-        m = fmax(m, partial[k]);
-    }
-    lds[threadIdx.x] = m;
-    __syncthreads();
-    for (unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + w]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        *out = lds[0];
-    }
 }
 
 } // namespace
@@ -554,6 +512,26 @@ __global__ __launch_bounds__(256) void reduce_partials(const double* partial, st
 template __global__ void reduce_partials<1>(const double*, std::uint32_t, double*);
 template __global__ void reduce_partials<6>(const double*, std::uint32_t, double*);
 template __global__ void reduce_partials<3>(const double*, std::uint32_t, double*);
+
+// Final stage of max_speed: the largest of `n` partial maxima.
+__global__ __launch_bounds__(256) void reduce_max(const double* partial, std::uint32_t n, double* out) {
+    __shared__ double lds[256];
+    double m = 0.0;
+    for (std::uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        m = fmax(m, partial[k]);
+    }
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (unsigned w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out = lds[0];
+    }
+}
 
 // ---- launchers ---------------------------------------------------------------------------------
 namespace {

@@ -157,4 +157,27 @@ __device__ __forceinline__ std::uint32_t xcd_tile(std::uint32_t b, std::uint32_t
     return xcd * q + (xcd < r ? xcd : r) + local;
 }
 
+// Largest |u| of one block (exact: a maximum does not depend on the order it is taken in).
+__device__ __forceinline__ void block_max(double v, double* partial) {
+    __shared__ double lds[16];
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        v = fmax(v, __shfl_down(v, offset, 64));
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        const int waves = static_cast<int>(blockDim.x >> 6);
+        for (int w = 0; w < waves; ++w) {
+            m = fmax(m, lds[w]);
+        }
+        partial[blockIdx.x] = m;
+    }
+}
+
 } // namespace lbm

@@ -61,6 +61,14 @@ hipError_t d2q9_populations(const Lattice2<typename T::R>& p, int phase, double*
 template <class T>
 hipError_t d2q9_moments(const Lattice2<typename T::R>& p, int phase, const double force[2], double* out,
                         hipStream_t s);
+// Moments of n listed nodes (slot: boundary slot or -1): out = rho[n], ux[n], uy[n].
+template <class T>
+hipError_t d2q9_sample(const Lattice2<typename T::R>& p, int phase, const double force[2], const std::uint32_t* cells,
+                       const std::int32_t* slots, std::uint32_t n, double* out, hipStream_t s);
+// Largest |u| over the fluid nodes into *result; partial as for d2q9_mass.
+template <class T>
+hipError_t d2q9_max_speed(const Lattice2<typename T::R>& p, int phase, const double force[2], double* partial,
+                          double* result, hipStream_t s);
 template <class T>
 hipError_t d2q9_mass(const Lattice2<typename T::R>& p, int phase, double* partial, double* out, hipStream_t s);
 // partial: 3 * reduction_blocks(nlinks) doubles; out: 3 doubles (fx, fy, moment).

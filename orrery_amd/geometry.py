@@ -184,6 +184,16 @@ def naca_wing(max_camber: float, camber_position: float, thickness: float, chord
                                                          span_divisions, out))
 
 
+def naca_section(max_camber: float, camber_position: float, thickness: float, n: int) -> np.ndarray:
+    """Naca4::section(n) (src/geom/naca.cpp:155-172): [2 n, 2] points (x, z) per unit chord,
+    counter-clockwise from the trailing edge over the upper surface."""
+    lib = _lib()
+    out = np.empty((2 * max(int(n), 0), 2))
+    _check(lib, lib.lbm_naca_section(max_camber, camber_position, thickness, int(n),
+                                     out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if out.size else None))
+    return out
+
+
 @dataclass
 class GridSpec:
     """Cell (i, j, k) at index i + nx (j + ny k), centre origin + (i + 1/2, j + 1/2, k + 1/2) spacing."""
@@ -307,6 +317,6 @@ def rotation_z(angle: float) -> np.ndarray:
 
 
 __all__ = ["TriangleMesh", "RevolutionResolution", "CapsuleShape", "sphere_mesh", "cylinder_mesh", "cone_mesh",
-           "capsule_mesh", "wedge_mesh", "box_mesh", "naca_wing", "GridSpec", "VoxelGrid", "SignedDistanceField",
+           "capsule_mesh", "wedge_mesh", "box_mesh", "naca_wing", "naca_section", "GridSpec", "VoxelGrid", "SignedDistanceField",
            "voxelize", "signed_distance_band", "place_voxels", "place_distance", "rotation_y", "rotation_z",
            "FAR_DISTANCE", "GeometryError"]

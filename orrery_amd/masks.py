@@ -127,3 +127,18 @@ def mask_from_sdf(nx: int, ny: int, sdf) -> SolidMask2D:
 def circle_mask(nx: int, ny: int, cx: float, cy: float, radius: float) -> SolidMask2D:
     """Circle of radius r centred at (cx, cy) (circle_mask, src/aero/lbm/solid_mask.cpp:39-43)."""
     return mask_from_sdf(nx, ny, lambda x, y: np.hypot(x - float(cx), y - float(cy)) - float(radius))
+
+
+def polygon_mask(nx: int, ny: int, polygon) -> SolidMask2D:
+    """Nodes inside a closed polygon [n, 2] of (x, z) points in lattice units, even-odd rule
+    (polygon_mask, src/aero/lbm/solid_mask.cpp:45-73), computed by the library."""
+    import ctypes
+
+    from . import abi
+    lib = abi.load()
+    pts = np.ascontiguousarray(np.asarray(polygon, dtype=np.float64).reshape(-1, 2))
+    mask = fluid_mask(nx, ny)
+    if lib.lbm_polygon_mask(nx, ny, abi.ptr(pts, ctypes.c_double), len(pts),
+                            abi.ptr(mask.solid, ctypes.c_uint8)) != abi.LBM_OK:
+        raise ValueError(lib.lbm_geom_last_error().decode())
+    return mask

@@ -6,17 +6,18 @@ with its result out, and a solver thread driven by a command queue.
   quantities (src/geom/reference_quantities.cpp: wetted area, silhouette areas on a 512-cell
   raster, extent along the flow, volume centroid) and moment centre;
 - run_experiment restates solve + record_result (src/app/aero/aero_solve.cpp:166-261) for the
-  3D lattice Boltzmann solver: options from the record's settings, steps until the solver
-  finishes, and the AeroResultRecord (coefficients, Reynolds and Mach numbers, reference
-  values, iterations, convergence, warning codes, residuals, force history);
+  lattice Boltzmann solvers (3D bodies on LbmAeroSolver3D, 2D NACA sections on
+  LbmAeroSolver2D): options from the record's settings, steps until the solver finishes, and
+  the AeroResultRecord (coefficients, Reynolds and Mach numbers, reference values, iterations,
+  convergence, warning codes, residuals, force history, Strouhal number of a section);
 - ExperimentRunner restates the command protocol of AeroRunner (src/app/aero/aero_runner.h):
   start, pause, steps per frame, run to convergence, one frame, stop; progress published after
   each change;
 - serve_request is the JSON request/response form: an experiment record in, the same record
   with its result out.
 
-The records' other solvers (panel, Newtonian, "auto"), 2D sections and imported models are not
-part of this library: such records are refused with a message.
+The records' other solvers (panel, Newtonian, "auto") and imported models are not part of this
+library: such records are refused with a message.
 """
 from __future__ import annotations
 
@@ -31,14 +32,16 @@ from typing import Optional
 import numpy as np
 
 from . import geometry as geo
-from .aero import AeroGeometry, LbmAeroSolver3D, LbmSolver3DOptions, ReferenceValues
+from .aero import AeroError, AeroGeometry, LbmAeroSolver2D, LbmAeroSolver3D, LbmSolver3DOptions, LbmSolverOptions, ReferenceValues
 from .experiment import (AeroExperiment, AeroResultRecord, AeroShape, GeometryRef, GeometrySource, ReferenceAreaKind,
                          SolverKind, parse_aero_experiment, save_aero_experiment)
 
 REVOLUTION = geo.RevolutionResolution(48, 0.08)  # kRevolutionAround, kRevolutionArcStep
 NACA_STATIONS = 48
 NACA_SPAN_DIVISIONS = 12
+SECTION_PREVIEW_SPAN = 0.6  # kSectionPreviewSpan: span of a section's preview wing, chords
 REFERENCE_RASTER = 512
+STEPS_PER_CALL_2D = 50
 STEPS_PER_CALL_3D = 20
 MAX_CELLS_3D = 1 << 22
 
@@ -66,6 +69,8 @@ class PreparedGeometry:
     quantities: ReferenceQuantities
     moment_centre: tuple
     label: str
+    section: Optional[np.ndarray] = None  # (x, z) points of a 2D section, metres
+    two_dimensional: bool = False
 
 
 def _rotated(mesh: geo.TriangleMesh, rows) -> geo.TriangleMesh:
@@ -194,6 +199,16 @@ def prepare_geometry(ref: GeometryRef) -> PreparedGeometry:
                              d.span_m, NACA_STATIONS, NACA_SPAN_DIVISIONS)
         label = f"NACA {digits} wing, chord {_metres(d.chord_m)}, span {_metres(d.span_m)}"
         naca = True
+    elif s == AeroShape.naca_section:
+        # naca_section_geometry (src/aero/panel2d.cpp:145-160): the section scaled by the chord,
+        # moment centre at the quarter chord; a short wing of the section for display.
+        digits = ref.naca
+        m, p, t = int(digits[0]) / 100.0, int(digits[1]) / 10.0, int(digits[2:]) / 100.0
+        section = geo.naca_section(m, p, t, NACA_STATIONS) * d.chord_m
+        mesh = geo.naca_wing(m, p, t, d.chord_m, SECTION_PREVIEW_SPAN * d.chord_m, NACA_STATIONS, 1)
+        label = f"NACA {digits} section, chord {_metres(d.chord_m)}"
+        return PreparedGeometry(ref, mesh, reference_quantities(mesh), (0.25 * d.chord_m, 0.0, 0.0), label,
+                                section=section, two_dimensional=True)
     else:
         raise RunnerError(f"the {s.value} primitive is not built by this library (sphere, cube, cylinder, cone, "
                           "flat_plate, wedge, capsule and naca_wing are)")
@@ -208,7 +223,12 @@ def prepare_geometry(ref: GeometryRef) -> PreparedGeometry:
 
 
 def reference_values(prepared: PreparedGeometry, settings) -> ReferenceValues:
-    """aero_geometry.cpp:239-255 (3D)"""
+    """aero_geometry.cpp:239-255: per unit span for a section (chord x 1 m, the chord as the
+    length), else the record's reference area of the body."""
+    if prepared.two_dimensional:
+        chord = prepared.ref.dimensions.chord_m
+        length = settings.reference_length_m if settings.reference_length_m is not None else chord
+        return ReferenceValues(chord, length, prepared.moment_centre)
     q = prepared.quantities
     area = {ReferenceAreaKind.frontal: q.frontal_area, ReferenceAreaKind.planform: q.planform_area,
             ReferenceAreaKind.wetted: q.wetted_area}[settings.reference_area]
@@ -223,8 +243,6 @@ def _check_runnable(e: AeroExperiment) -> None:
         raise RunnerError(f"solver '{e.solver.solver.value}' is not part of this library (lattice Boltzmann only)")
     if e.solver.solver is None:
         raise RunnerError("solver 'auto' picks among solvers this library does not have: name 'lbm'")
-    if e.geometry.source == GeometrySource.primitive and e.geometry.shape == AeroShape.naca_section:
-        raise RunnerError("2D sections run on the D2Q9 lattice through Lbm2DSolver, not through this runner")
 
 
 def solver_options(e: AeroExperiment, storage: str = "f32",
@@ -240,20 +258,66 @@ def solver_options(e: AeroExperiment, storage: str = "f32",
     return o
 
 
+def solver_options_2d(e: AeroExperiment, storage: str = "f32",
+                      steps_per_call: int = STEPS_PER_CALL_2D) -> LbmSolverOptions:
+    """make_solver for the 2D lattice (aero_solve.cpp:91-98)."""
+    o = LbmSolverOptions(storage=storage)
+    if e.solver.lbm_cells_per_length > 0.0:
+        o.cells_per_length = e.solver.lbm_cells_per_length
+    if e.solver.lbm_duration > 0.0:
+        o.duration = e.solver.lbm_duration
+    o.steps_per_call = steps_per_call
+    return o
+
+
+def steps_per_call_for(two_dimensional: bool, limit: Optional[int] = None) -> int:
+    """lbm_steps_per_call_for (aero_solve.cpp:57-70): the per-call batch, or its largest divisor
+    of a step limit so that a limited run stops on it."""
+    batch = STEPS_PER_CALL_2D if two_dimensional else STEPS_PER_CALL_3D
+    if not limit:
+        return batch
+    for n in range(batch, 1, -1):
+        if limit % n == 0:
+            return n
+    return 1
+
+
+def make_solver(prepared: PreparedGeometry, e: AeroExperiment, storage: str = "f32",
+                steps_per_call: Optional[int] = None):
+    """The adapter of a record's geometry with its options (make_solver, aero_solve.cpp:72-111)."""
+    if prepared.two_dimensional:
+        per_call = steps_per_call if steps_per_call else STEPS_PER_CALL_2D
+        return LbmAeroSolver2D(solver_options_2d(e, storage, per_call))
+    per_call = steps_per_call if steps_per_call else STEPS_PER_CALL_3D
+    return LbmAeroSolver3D(solver_options(e, storage, per_call))
+
+
+def aero_geometry(prepared: PreparedGeometry, reference: ReferenceValues) -> AeroGeometry:
+    """to_aero_geometry (aero_geometry.cpp:257-268): the section of a 2D record, else the mesh."""
+    if prepared.two_dimensional:
+        return AeroGeometry(reference=reference, section=prepared.section)
+    return AeroGeometry(prepared.mesh, reference)
+
+
 def _finite(x: float) -> bool:
     return math.isfinite(x)
 
 
-def record_result(solver: LbmAeroSolver3D, e: AeroExperiment, reference: ReferenceValues) -> AeroResultRecord:
+def record_result(solver, e: AeroExperiment, reference: ReferenceValues) -> AeroResultRecord:
     """record_result (aero_solve.cpp:166-211) of a finished or interrupted run."""
     r = solver.result()
     flow = e.conditions.flow()
     t, cd, cl = solver.history()
+    two_d = isinstance(solver, LbmAeroSolver2D)
+    warnings = [w.code for w in r.warnings]
+    if two_d and e.conditions.sideslip_deg != 0.0:
+        warnings.append("workspace.sideslip_2d")  # summarize_flow (aero_flow.cpp:82-85)
     out = AeroResultRecord(solver=SolverKind.lbm, cd=r.cd, cl=r.cl, cy=r.cy, cm=r.cm,
                            reynolds=flow.reynolds(reference.length), mach=flow.mach(),
                            reference_area_m2=reference.area, reference_length_m=reference.length,
                            iterations=int(r.iterations), converged=bool(r.converged),
-                           warnings=[w.code for w in r.warnings], residuals=[float(x) for x in r.residuals],
+                           strouhal=solver.strouhal() if two_d else None,
+                           warnings=warnings, residuals=[float(x) for x in r.residuals],
                            history_step=[float(x) for x in t], history_cd=[float(x) for x in cd],
                            history_cl=[float(x) for x in cl])
     for key in ("cd", "cl", "cy", "cm"):
@@ -268,20 +332,24 @@ def record_result(solver: LbmAeroSolver3D, e: AeroExperiment, reference: Referen
 
 
 def run_experiment(e: AeroExperiment, storage: str = "f32", max_lattice_steps: Optional[int] = None,
-                   steps_per_call: int = STEPS_PER_CALL_3D) -> AeroExperiment:
+                   steps_per_call: Optional[int] = None) -> AeroExperiment:
     """solve (aero_solve.cpp:226-261): the record with its result."""
     _check_runnable(e)
     prepared = prepare_geometry(e.geometry)
     reference = reference_values(prepared, e.solver)
-    solver = LbmAeroSolver3D(solver_options(e, storage, steps_per_call))
+    if steps_per_call is None:
+        steps_per_call = steps_per_call_for(prepared.two_dimensional, max_lattice_steps)
+    solver = make_solver(prepared, e, storage, steps_per_call)
     try:
-        solver.setup(AeroGeometry(prepared.mesh, reference), e.conditions.flow())
+        solver.setup(aero_geometry(prepared, reference), e.conditions.flow())
         finished = False
         while not finished:
             finished = solver.step()
             if max_lattice_steps is not None and solver.result().iterations >= max_lattice_steps:
                 break
         out = dataclasses.replace(e, result=record_result(solver, e, reference))
+    except (AeroError, RuntimeError) as err:
+        raise RunnerError(f"lbm: {err}") from err
     finally:
         solver.close()
     return out
@@ -364,7 +432,8 @@ class ExperimentRunner:
         self._lock = threading.Lock()
         self._progress = Progress(RunStatus())
         self._status = RunStatus()
-        self._solver: Optional[LbmAeroSolver3D] = None
+        self._solver = None  # LbmAeroSolver3D or LbmAeroSolver2D
+        self._per_call = STEPS_PER_CALL_3D
         self._experiment: Optional[AeroExperiment] = None
         self._reference: Optional[ReferenceValues] = None
         self._stop = threading.Event()
@@ -431,10 +500,15 @@ class ExperimentRunner:
             _check_runnable(cmd.experiment)
             prepared = prepare_geometry(cmd.experiment.geometry)
             self._reference = reference_values(prepared, cmd.experiment.solver)
-            self._solver = LbmAeroSolver3D(solver_options(cmd.experiment, self._storage))
-            self._solver.setup(AeroGeometry(prepared.mesh, self._reference), cmd.experiment.conditions.flow())
-        except Exception as err:  # setup refusals end the run with their message
+            self._per_call = STEPS_PER_CALL_2D if prepared.two_dimensional else STEPS_PER_CALL_3D
+            self._solver = make_solver(prepared, cmd.experiment, self._storage)
+        except Exception as err:  # refusals before the solver exists end the run with their message
             self._fail(str(err))
+            return
+        try:
+            self._solver.setup(aero_geometry(prepared, self._reference), cmd.experiment.conditions.flow())
+        except Exception as err:  # setup refusals (AeroRunner::start, aero_runner.cpp:205-211)
+            self._fail(f"Lattice Boltzmann setup failed: {err}")
             return
         self._status.state = RunState.running
         self._publish(False)
@@ -483,7 +557,7 @@ class ExperimentRunner:
         return True
 
     def _frame(self) -> None:
-        calls = max(1, -(-self._status.steps_per_frame // STEPS_PER_CALL_3D))
+        calls = max(1, -(-self._status.steps_per_frame // self._per_call))
         for _ in range(calls):
             if not self._one_call():
                 return
@@ -509,6 +583,7 @@ class ExperimentRunner:
 
 
 __all__ = ["prepare_geometry", "reference_quantities", "projected_area", "reference_values", "run_experiment",
-           "serve_request", "record_result", "solver_options", "ExperimentRunner", "StartRun", "SetPaused",
+           "serve_request", "record_result", "solver_options", "solver_options_2d", "steps_per_call_for",
+           "make_solver", "aero_geometry", "ExperimentRunner", "StartRun", "SetPaused",
            "SetStepsPerFrame", "SetToConvergence", "StepFrame", "StopRun", "RunState", "RunStatus", "Progress",
            "PreparedGeometry", "ReferenceQuantities", "RunnerError"]

@@ -310,6 +310,21 @@ class Lbm2DSolver:
         _check(self._lib, self._lib.lbm2d_macroscopic(self._h, *p))
         return MacroFields2D(c.nx, c.ny, out[0], out[1], out[2], np.asarray(self._mask.solid, dtype=np.uint8))
 
+    def sample(self, cells) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(rho, ux, uy) at the listed cell indices, computed on the device: the values
+        macroscopic() reports there."""
+        idx = np.ascontiguousarray(cells, dtype=np.uint32)
+        out = np.empty((3, len(idx)))
+        p = [abi.ptr(out[k], ctypes.c_double) for k in range(3)]
+        _check(self._lib, self._lib.lbm2d_sample(self._h, abi.ptr(idx, ctypes.c_uint32), len(idx), *p))
+        return out[0], out[1], out[2]
+
+    def max_speed(self) -> float:
+        """Largest hypot(ux, uy) over the fluid nodes, reduced on the device."""
+        v = ctypes.c_double()
+        _check(self._lib, self._lib.lbm2d_max_speed(self._h, ctypes.byref(v)))
+        return v.value
+
     def populations(self) -> np.ndarray:
         f = np.empty(9 * self._config.cells)
         _check(self._lib, self._lib.lbm2d_populations(self._h, abi.ptr(f, ctypes.c_double)))

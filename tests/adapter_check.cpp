@@ -1,6 +1,7 @@
 // Drives orrery_amd::make_hip_solver (include/orrery_lbm_solver.hpp) through the reference's own
 // Lbm3DSolver types: prints the refusal message of each configuration named on the command line,
 // or "created" when the solver was made. Built by tests/test_adapter.py.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -36,6 +37,38 @@ int main(int argc, char** argv) {
                 o.resolution.lattice_velocity = 0.06;
             }
             orrery_amd::HipLbmAeroSolver3D solver(o);
+            try {
+                solver.setup(g, f);
+                std::printf("%s\tset up\n", argv[a]);
+            } catch (const std::invalid_argument& e) {
+                std::printf("%s\t%s\n", argv[a], e.what());
+            }
+            continue;
+        }
+        if (std::strncmp(argv[a], "aero2d-", 7) == 0) {
+            // The 2D IAeroSolver wrapper: a 48-gon of diameter 1, a refused flow or option.
+            orrery::aero::AeroGeometry g;
+            for (int k = 0; k < 48; ++k) {
+                const double t = 2.0 * 3.14159265358979323846 * k / 48.0;
+                g.section.push_back({0.5 * std::cos(t), 0.5 * std::sin(t)});
+            }
+            g.reference.length = 1.0;
+            g.reference.area = 1.0;
+            orrery::aero::FlowConditions f;
+            f.speed = 0.1;
+            f.density = 1000.0;
+            f.viscosity = 1.0;
+            ref::LbmSolverOptions o;
+            o.resolution.cells_per_length = 10.0;
+            const std::string what = argv[a] + 7;
+            if (what == "speed") {
+                f.speed = 0.0;
+            } else if (what == "section") {
+                g.section.resize(2);
+            } else if (what == "mach") {
+                o.resolution.lattice_velocity = 0.08;
+            }
+            orrery_amd::HipLbmAeroSolver2D solver(o);
             try {
                 solver.setup(g, f);
                 std::printf("%s\tset up\n", argv[a]);

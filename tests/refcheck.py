@@ -52,6 +52,9 @@ def load():
         "ref2d_zou_he_west": (None, [_f64p, ctypes.c_double, ctypes.c_double]),
         "ref2d_circle_mask": (None, [u64, u64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u8p]),
         "ref_lattice_tau": (ctypes.c_double, [ctypes.c_double] * 6),
+        "ref2d_polygon_mask": (None, [u64, u64, _f64p, ctypes.c_size_t, _u8p]),
+        "ref_naca_section": (None, [ctypes.c_double] * 3 + [ctypes.c_int, _f64p]),
+        "ref_naca_section_area": (ctypes.c_double, [ctypes.c_double] * 3),
         "ref_mesh_revolution": (ctypes.c_size_t, [ctypes.c_int] + [ctypes.c_double] * 5
                                 + [ctypes.c_int, ctypes.c_double, _f64p, ctypes.c_size_t, _u32p, ctypes.c_size_t,
                                    P(ctypes.c_size_t)]),
@@ -101,6 +104,23 @@ def circle_mask(nx, ny, cx, cy, r):
     solid = np.empty(nx * ny, dtype=np.uint8)
     load().ref2d_circle_mask(nx, ny, cx, cy, r, ptr(solid, ctypes.c_uint8))
     return solid
+
+
+def polygon_mask(nx, ny, points):
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 2))
+    solid = np.empty(nx * ny, dtype=np.uint8)
+    load().ref2d_polygon_mask(nx, ny, ptr(pts), len(pts), ptr(solid, ctypes.c_uint8))
+    return solid
+
+
+def naca_section(m, p, t, n):
+    out = np.empty((2 * n, 2))
+    load().ref_naca_section(m, p, t, n, ptr(out))
+    return out
+
+
+def naca_section_area(m, p, t):
+    return load().ref_naca_section_area(m, p, t)
 
 
 def boundary_links3(config, mask):

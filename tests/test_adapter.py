@@ -74,3 +74,16 @@ def test_aero_solver_wrapper_refuses_like_the_adapter(driver, what):
             "mach": "lattice units: lattice Mach number 0.103923 (u_lat 0.06) is not below 0.1; lower the lattice "
                     "velocity (weak compressibility, APPROXIMATION A3)"}[what]
     assert name == "aero-" + what and message == want
+
+
+@pytest.mark.parametrize("what", ["speed", "section", "mach"])
+def test_2d_aero_solver_wrapper_refuses_like_the_adapter(driver, what):
+    """HipLbmAeroSolver2D (include/orrery_lbm_aero.hpp) behind the reference's IAeroSolver: its
+    setup refusals are the 2D adapter's (the reference's texts, lbm_aero_solver.cpp:45-80)."""
+    out = subprocess.run([driver, "aero2d-" + what], check=True, capture_output=True, text=True).stdout.strip()
+    name, message = out.split("\t")
+    want = {"speed": "lbm: speed, density and viscosity must be positive and finite, angles finite",
+            "section": "lbm: the 2D solver needs a section of at least 3 points",
+            "mach": "lattice units: lattice Mach number 0.138564 (u_lat 0.08) is not below 0.1; lower the lattice "
+                    "velocity (weak compressibility, APPROXIMATION A3)"}[what]
+    assert name == "aero2d-" + what and message == want

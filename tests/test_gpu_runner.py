@@ -69,3 +69,42 @@ def test_runner_commands_reach_the_direct_result():
         assert r.wait({runner.RunState.idle}, timeout=60).status.state == runner.RunState.idle
     finally:
         r.close()
+
+
+def test_bundled_naca_section_record_runs_on_the_2d_lattice():
+    """The reference's bundled record assets/experiments/aero_naca0012_lbm.json (NACA 0012 at
+    4 degrees, Re 400, 40 cells per chord, 40 convective times) through the JSON request path:
+    the 2D adapter, per-span reference values, the history, and a positive lift."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "experiments", "aero_naca0012_lbm.json")
+    with open(path) as f:
+        text = f.read()
+    e = ex.parse_aero_experiment(runner.serve_request(text))
+    r = e.result
+    flow = e.conditions.flow()
+    print(f"measured: NACA 0012 section record: cl {r.cl:.4f} cd {r.cd:.4f} cm {r.cm:.4f} St {r.strouhal} "
+          f"converged {r.converged} iterations {r.iterations} warnings {r.warnings}", flush=True)
+    assert r.solver == ex.SolverKind.lbm and r.iterations == 32000
+    assert r.reference_area_m2 == 0.1 and r.reference_length_m == 0.1
+    assert r.reynolds == flow.reynolds(0.1) and r.mach == flow.mach()
+    assert 0.05 < r.cl < 0.6 and 0.02 < r.cd < 0.5
+    assert len(r.history_step) == len(r.history_cd) == 3200
+
+
+def test_section_record_through_the_runner_commands():
+    """A 2D record driven by the command thread reaches the direct run's record."""
+    req = REQUEST.replace('"shape": "sphere", "diameter_m": 0.2', '"shape": "naca_section", "naca": "2412", '
+                          '"chord_m": 0.1').replace('"lbm_cells_per_length": 8', '"lbm_cells_per_length": 16')
+    e = ex.parse_aero_experiment(req)
+    direct = runner.run_experiment(e).result
+    assert direct.iterations == 960 and direct.cl > 0.0
+    r = runner.ExperimentRunner()
+    try:
+        r.send(runner.StartRun(e, to_convergence=True))
+        p = r.wait({runner.RunState.finished, runner.RunState.failed}, timeout=300)
+        assert p.status.state == runner.RunState.finished, p.status.message
+        got = p.record.result
+        assert (got.cd, got.cl, got.cm, got.iterations) == (direct.cd, direct.cl, direct.cm, direct.iterations)
+        assert np.array_equal(got.history_cl, direct.history_cl)
+    finally:
+        r.close()

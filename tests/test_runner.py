@@ -93,13 +93,16 @@ def test_reference_values_follow_the_settings():
 @pytest.mark.parametrize("solver,shape,message", [
     (ex.SolverKind.panel, ex.AeroShape.sphere, "solver 'panel' is not part of this library"),
     (None, ex.AeroShape.sphere, "solver 'auto'"),
-    (ex.SolverKind.lbm, ex.AeroShape.naca_section, "2D sections"),
+    (ex.SolverKind.lbm, ex.AeroShape.naca_section, "lbm: the section covers no lattice node (resolution too low)"),
     (ex.SolverKind.lbm, ex.AeroShape.ellipsoid, "ellipsoid primitive is not built"),
 ])
 def test_unrunnable_records_fail_with_a_message(solver, shape, message):
+    """Refusals before the device lattice exists: the solver choice, the primitive, and the 2D
+    adapter's setup of a section too coarse to cover a node (0.01 cells per chord)."""
     e = ex.AeroExperiment(name="x")
     e.geometry.shape = shape
     e.solver.solver = solver
+    e.solver.lbm_cells_per_length = 0.01
     with pytest.raises(runner.RunnerError) as err:
         runner.run_experiment(e)
     assert message in str(err.value)
