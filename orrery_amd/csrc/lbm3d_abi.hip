@@ -248,6 +248,8 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
         ((h->tile_slot = static_cast<std::uint32_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, 19U * static_cast<std::size_t>(t.bstride) * 4U, "source codes")) ||
         ((h->bcode = static_cast<std::uint32_t*>(tmp)), false) ||
+        (rc = alloc(*h, &tmp, static_cast<std::size_t>(t.bstride) * 4U, "lattice write masks")) ||
+        ((h->bwrite = static_cast<std::uint32_t*>(tmp)), false) ||
         (interpolated && (rc = alloc(*h, &h->bq, 19U * static_cast<std::size_t>(t.bstride) * rb, "wall buffer"))) ||
         (rc = alloc(*h, &h->stash, 2U * 19U * static_cast<std::size_t>(t.bstride) * rb, "boundary stash")) ||
         (rc = alloc(*h, &h->hist, 5U * static_cast<std::size_t>(t.bstride) * rb, "outlet buffer")) ||
@@ -284,7 +286,8 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     }
     if ((rc = upload(*h, h->cls, t.cls, "node class upload")) || (rc = upload(*h, h->bcell, t.bcell, "boundary upload")) ||
         (rc = upload(*h, h->tile_slot, t.tile_slot, "boundary slot upload")) ||
-        (rc = upload(*h, h->bcode, t.bcode, "code upload")) || (rc = upload(*h, h->lcell, lcell, "link upload")) ||
+        (rc = upload(*h, h->bcode, t.bcode, "code upload")) || (rc = upload(*h, h->bwrite, t.bwrite, "mask upload")) ||
+        (rc = upload(*h, h->lcell, lcell, "link upload")) ||
         (rc = upload(*h, h->ldir, ldir, "link upload")) || (rc = upload(*h, h->lq, lq, "link upload")) ||
         (rc = upload(*h, h->lslot, t.lslot, "link upload"))) {
         release(h);

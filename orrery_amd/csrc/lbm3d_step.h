@@ -486,8 +486,11 @@ __device__ __forceinline__ std::uint32_t mirrored(int i, std::uint32_t code) {
 }
 
 // ---- one node of the step: boundary half -----------------------------------------------------------------
-// One boundary node. Sources come from the precomputed codes; values other than
-// the node's own natural pull come from the stash of the previous step.
+// One boundary node. Sources come from the precomputed codes; values other than the natural
+// pulls from bulk nodes come from the stash of the previous step (a boundary node's stash slot
+// holds exactly what it stored into the lattice, so a boundary source is read there, densely,
+// instead of from a row-scattered lattice address). The node stores into the lattice only the
+// outputs a lattice reader will load (bwrite).
 template <class T, bool kLocal>
 __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, std::uint32_t b) {
     using S = typename T::S;
@@ -544,7 +547,7 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
             const std::uint32_t src = far ? static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane)) : payload;
             fv[i] = (pull || far) ? f[(far ? i : L3::opp(i)) * stride + src] : S(0);
         }
-        const bool other = (kind == kNode && !plain) || kind == kOutlet;
+        const bool other = (kind == kNode && !plain) || kind == kOutlet || kind == kStash;
         sv[i] = other ? st[mirrored(i, code[i]) * bs + payload] : R(0);
         q[i] = (far && p.bq != nullptr) ? p.bq[i * bs + b] : R(0.5);
     }
@@ -572,6 +575,8 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
         const std::uint32_t payload = code[i] >> kCodeShift;
         if (kind == kNode) {
             v[i] = (code[i] & (kCodeMirrorY | kCodeMirrorZ)) == 0U ? T::load(fv[i]) : sv[i];
+        } else if (kind == kStash) {
+            v[i] = sv[i];
         } else if (kind == kBounce) {
             // reflected (lbm3d_internal.h:185-197)
             const R leaving = own[L3::opp(i)];
@@ -602,10 +607,14 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
     }
     const R sponge = p.sponge_strength > R(0) ? sponge_sigma3(p, x, y, z) : R(0);
     collide3<T>(v, p, sponge);
+    const std::uint32_t wmask = p.bwrite[b];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const S packed = T::pack(v[i]);
         sw[i * bs + b] = T::load(packed);
+        if (((wmask >> i) & 1U) == 0U) {
+            continue;
+        }
         if constexpr (kLocal) {
             f[L3::opp(i) * stride + cell] = packed;
         } else {

@@ -83,8 +83,8 @@ struct StoreF64 {
 // ---- boundary-node source codes -------------------------------------------------------------
 // One 32-bit word per (boundary node, direction), built on the host from resolve_d3q19:
 //   bits 0..2 kind (SourceKind), bit 3 mirror_y applied, bit 4 mirror_z applied,
-//   bits 5..31 payload: kNode unmirrored -> source cell; kNode mirrored / kOutlet -> source
-//   boundary slot; kWall -> side (0..3).
+//   bits 5..31 payload: kNode unmirrored -> source cell; kNode mirrored / kOutlet / kStash ->
+//   source boundary slot; kWall -> side (0..3).
 constexpr std::uint32_t kCodeMirrorY = 8U;
 constexpr std::uint32_t kCodeMirrorZ = 16U;
 constexpr int kCodeShift = 5;
@@ -101,6 +101,7 @@ template <typename R> struct Lattice3 {
     const std::uint8_t* cls;  // node class per cell: 0 solid, 1 bulk, 2 boundary
     const std::uint32_t* bcell;  // boundary node -> cell [nb]
     const std::uint32_t* bcode;  // [19][bstride]
+    const std::uint32_t* bwrite; // [bstride] bit i: the lattice copy of output i has a reader
     const R* bq;                 // [19][bstride] wall fractions (interpolated wall) or null
     R* stash;                    // [2][19][bstride]
     R* hist;                     // [5][bstride] convective-outlet history (outlet nodes)

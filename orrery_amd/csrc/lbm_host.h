@@ -39,6 +39,10 @@ struct Tables {
     std::vector<std::uint8_t> cls;       // 0 solid, 1 bulk, 2 boundary
     std::vector<std::uint32_t> bcell;    // boundary node -> cell (cell order)
     std::vector<std::uint32_t> bcode;    // [Q][bstride] source codes
+    // D3Q19: per boundary node, bit i set when its output i must also be stored in the lattice
+    // (the node it streams to reads the lattice there: a bulk node, or the far read of a short
+    // bounce-back link); boundary readers of boundary sources read the stash instead.
+    std::vector<std::uint32_t> bwrite;
     std::vector<double> bq;              // [Q][bstride] wall fractions (interpolated wall only)
     std::vector<Link> links;             // boundary_links order
     std::vector<std::uint32_t> lslot;    // boundary slot of each link's node

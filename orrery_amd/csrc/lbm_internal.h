@@ -43,6 +43,7 @@ struct lbm3d {
     std::uint8_t* cls = nullptr;
     std::uint32_t* bcell = nullptr;
     std::uint32_t* bcode = nullptr;
+    std::uint32_t* bwrite = nullptr;
     void* bq = nullptr;
     void* stash = nullptr;
     void* hist = nullptr;
@@ -97,6 +98,7 @@ inline std::size_t lattice_bytes(std::size_t cells, std::uint32_t stride, std::s
     total += t.bcell.size() * 4U;                                       // boundary list
     total += t.tile_slot.size() * 4U;                                   // boundary slot per 64 cells
     total += 19U * static_cast<std::size_t>(t.bstride) * 4U;            // source codes
+    total += static_cast<std::size_t>(t.bstride) * 4U;                  // lattice write masks
     if (interpolated) {
         total += 19U * static_cast<std::size_t>(t.bstride) * rb; // wall fractions
     }
@@ -127,6 +129,7 @@ template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
     p.cls = h.cls;
     p.bcell = h.bcell;
     p.bcode = h.bcode;
+    p.bwrite = h.bwrite;
     p.bq = static_cast<const R*>(h.bq);
     p.stash = static_cast<R*>(h.stash);
     p.hist = static_cast<R*>(h.hist);

@@ -79,6 +79,9 @@ struct D2Q9 {
 // Kinds of population source in the pull formulation of one step (the reference's SourceKind,
 // src/aero/lbm3d/lbm3d_internal.h:20-28).
 enum SourceKind : int { kNode = 0, kBounce = 1, kWall = 2, kInlet = 3, kOutlet = 4 };
+// Device code only (D3Q19): a natural pull whose source is itself a boundary node, read from
+// that node's stash slot instead of the lattice.
+constexpr int kStash = 5;
 
 enum WallCode : int { kPeriodic = 0, kSlip = 1, kNoSlip = 2 };
 

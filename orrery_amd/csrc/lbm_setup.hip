@@ -388,10 +388,18 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
             const bool my = (r.mirror & 1) != 0;
             const bool mz = (r.mirror & 2) != 0;
             switch (r.kind) {
-                case kNode:
-                    code = r.mirror == 0 ? make_code(kNode, false, false, cell_of(r.sx, r.sy, r.sz))
-                                         : make_code(kNode, my, mz, slot_of[cell_of(r.sx, r.sy, r.sz)]);
+                case kNode: {
+                    const std::uint32_t src = cell_of(r.sx, r.sy, r.sz);
+                    if (r.mirror != 0) {
+                        code = make_code(kNode, my, mz, slot_of[src]);
+                    } else if (t.cls[src] == 2) {
+                        // The source's output i is also in its stash slot, densely.
+                        code = make_code(kStash, false, false, slot_of[src]);
+                    } else {
+                        code = make_code(kNode, false, false, src);
+                    }
                     break;
+                }
                 case kBounce: {
                     code = make_code(kBounce, false, false, 0);
                     // boundary_links order: z, y, x, direction (i >= 1 always: c_0 = 0)
@@ -416,6 +424,57 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
             }
             t.bcode[static_cast<std::size_t>(i) * t.bstride + b] = code;
         }
+    }
+    // Which outputs of a boundary node need their lattice copy: output i of node w streams to
+    // n = w + c_i (through periodic sides); n reads it from the lattice when n is a bulk node,
+    // or when n is a boundary node whose link opposite(i) is a bounce-back (the far read of a
+    // short link, also made by the force). Every other reader is a boundary node reading w's
+    // stash (kStash), and targets outside the domain or solid have no reader.
+    t.bwrite.assign(t.bstride, 0U);
+    for (std::size_t b = 0; b < nb; ++b) {
+        const std::size_t cell = t.bcell[b];
+        const int z = static_cast<int>(cell / plane);
+        const int y = static_cast<int>((cell % plane) / nx);
+        const int x = static_cast<int>(cell % nx);
+        std::uint32_t mask = 0;
+        for (int i = 0; i < 19; ++i) {
+            int tx = x + D3Q19::cx(i);
+            int ty = y + D3Q19::cy(i);
+            int tz = z + D3Q19::cz(i);
+            bool inside = true;
+            if (ty < 0 || ty >= iny) {
+                if (sides[ty < 0 ? 0 : 1] == LBM_WALL_PERIODIC) {
+                    ty = ty < 0 ? ty + iny : ty - iny;
+                } else {
+                    inside = false;
+                }
+            }
+            if (tz < 0 || tz >= inz) {
+                if (sides[tz < 0 ? 2 : 3] == LBM_WALL_PERIODIC) {
+                    tz = tz < 0 ? tz + inz : tz - inz;
+                } else {
+                    inside = false;
+                }
+            }
+            if (tx < 0 || tx >= inx) {
+                if (c.x_boundary == LBM_X_PERIODIC) {
+                    tx = tx < 0 ? tx + inx : tx - inx;
+                } else {
+                    inside = false;
+                }
+            }
+            if (!inside) {
+                continue;
+            }
+            const std::uint32_t n = cell_of(tx, ty, tz);
+            bool needed = t.cls[n] == 1;
+            if (t.cls[n] == 2) {
+                const std::uint32_t far = t.bcode[static_cast<std::size_t>(D3Q19::opp(i)) * t.bstride + slot_of[n]];
+                needed = static_cast<int>(far & 7U) == kBounce;
+            }
+            mask |= needed ? 1U << i : 0U;
+        }
+        t.bwrite[b] = mask;
     }
     return t;
 }
