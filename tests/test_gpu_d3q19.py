@@ -15,7 +15,7 @@ from cases import (MAGIC_TAU, couette3d_config, duct_config, l2_relative, period
                    sphere_channel, uniform_state3d, uniform_state_error, velocity_l2, wall)
 from oracle import Oracle3D
 from orrery_amd import (Lbm3DSolver, LbmError, Outlet, Smagorinsky, SolidWall, Sponge, WallKind, XBoundary, box_mask,
-                        fluid_mask_3d)
+                        boundary_links, fluid_mask_3d)
 
 pytestmark = pytest.mark.gpu
 
@@ -104,6 +104,46 @@ def test_every_wall_kind_f64_matches_the_oracle_to_rounding():
     c.sponge = Sponge(0.1, 4, 6, 3)
     for steps in (1, 2, 77):
         out, cpu, gpu = compare(c, mask, steps, "f64", f"all walls {steps}")
+        assert out["populations"] < 1e-13
+        assert out["velocity"] < 1e-12
+        assert out["force_x"] < 1e-10
+
+
+def _two_spheres(nx, ny, nz, a, ra, b, rb):
+    from orrery_amd.masks import mask_from_sdf_3d
+
+    def sdf(x, y, z):
+        da = np.sqrt((x - a[0]) ** 2 + (y - a[1]) ** 2 + (z - a[2]) ** 2) - ra
+        db = np.sqrt((x - b[0]) ** 2 + (y - b[1]) ** 2 + (z - b[2]) ** 2) - rb
+        return np.minimum(da, db)
+
+    return mask_from_sdf_3d(nx, ny, nz, sdf)
+
+
+@pytest.mark.parametrize("case", ["periodic_seams", "face_adjacent"])
+def test_boundary_sources_and_lattice_stores_f64_match_the_oracle(case):
+    """Boundary nodes read other boundary nodes' outputs from the stash and store into the
+    lattice only what a lattice reader loads. The readers that decide it meet here: bodies whose
+    short interpolated links (q < 1/2, far reads) sit next to the periodic seams, the inlet and
+    outlet faces and the moving walls, with stash sources through every periodic wrap; odd and
+    even step counts, against the oracle to rounding."""
+    if case == "periodic_seams":
+        c = periodic_box(20, 14, 12, 0.56)
+        c.z_low = wall(WallKind.slip)
+        c.z_high = wall(WallKind.no_slip, 0.02)
+        c.body_force_x = 2e-5
+        mask = _two_spheres(20, 14, 12, (1.3, 0.7, 6.2), 3.1, (17.6, 12.4, 3.3), 2.2)
+    else:
+        c, _ = sphere_channel(30, 14, 12, 6.0, 2.0, 0.04, 30.0)
+        c.z_low = wall(WallKind.no_slip, 0.01)
+        c.y_low = wall(WallKind.periodic)
+        c.y_high = wall(WallKind.periodic)
+        mask = _two_spheres(30, 14, 12, (2.4, 1.2, 5.6), 1.45, (27.2, 6.7, 1.4), 1.7)
+    c.smagorinsky = Smagorinsky(True, 0.12)
+    c.solid_wall = SolidWall.interpolated
+    assert (boundary_links(c, mask).q < 0.5).sum() > 0  # far reads are exercised
+    for steps in (1, 2, 61):
+        out, cpu, gpu = compare(c, mask, steps, "f64", f"{case} {steps}")
         assert out["populations"] < 1e-13
         assert out["velocity"] < 1e-12
         assert out["force_x"] < 1e-10
