@@ -45,6 +45,33 @@ int main(int argc, char** argv) {
             }
             continue;
         }
+        if (std::strncmp(argv[a], "lattice2d-", 10) == 0) {
+            // make_hip_solver for the D2Q9 lattice: a 16 x 8 channel, one parameter refused.
+            ref::Lbm2DConfig c;
+            c.nx = 16;
+            c.ny = 8;
+            c.tau = 0.6;
+            c.inlet_ux = 0.05;
+            ref::SolidMask2D m;
+            m.nx = 16;
+            m.ny = 8;
+            m.solid.assign(16 * 8, 0);
+            const std::string what = argv[a] + 10;
+            if (what == "tau") {
+                c.tau = 0.5;
+            } else if (what == "mach") {
+                c.inlet_ux = 0.2;
+            } else if (what == "size") {
+                m.nx = 15;
+            } else if (what == "periodic") {
+                c.top.kind = ref::WallKind::periodic;
+            } else if (what == "inlet") {
+                m.solid[16 * 3] = 1;
+            }
+            auto made = orrery_amd::make_hip_solver(c, m);
+            std::printf("%s\t%s\n", argv[a], made ? "created" : made.error().c_str());
+            continue;
+        }
         if (std::strncmp(argv[a], "aero2d-", 7) == 0) {
             // The 2D IAeroSolver wrapper: a 48-gon of diameter 1, a refused flow or option.
             orrery::aero::AeroGeometry g;

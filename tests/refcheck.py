@@ -154,6 +154,14 @@ def validate3(config, mask):
     return buf.value.decode() if rc else None
 
 
+def validate2(config, mask):
+    c = config.to_c()
+    buf = ctypes.create_string_buffer(512)
+    rc = load().ref2d_validate(ctypes.byref(c), ptr(np.ascontiguousarray(mask.solid, dtype=np.uint8), ctypes.c_uint8),
+                               buf, 512)
+    return buf.value.decode() if rc else None
+
+
 def boundary_links2(config, mask):
     lib = load()
     c = config.to_c()

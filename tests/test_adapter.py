@@ -87,3 +87,30 @@ def test_2d_aero_solver_wrapper_refuses_like_the_adapter(driver, what):
             "mach": "lattice units: lattice Mach number 0.138564 (u_lat 0.08) is not below 0.1; lower the lattice "
                     "velocity (weak compressibility, APPROXIMATION A3)"}[what]
     assert name == "aero2d-" + what and message == want
+
+
+@pytest.mark.parametrize("what", ["tau", "mach", "periodic", "inlet"])
+def test_2d_lattice_refusals_are_the_references(driver, what):
+    """make_hip_solver for Lbm2DConfig (HipLbm2D, include/orrery_lbm_solver.hpp) refuses with
+    the reference's validate() messages (oracle/_ref)."""
+    from orrery_amd import Lbm2DConfig, fluid_mask
+    from orrery_amd.config import Wall, WallKind
+    out = subprocess.run([driver, "lattice2d-" + what], check=True, capture_output=True, text=True).stdout.strip()
+    name, message = out.split("\t")
+    c = Lbm2DConfig(nx=16, ny=8, tau=0.6, inlet_ux=0.05)
+    m = fluid_mask(16, 8)
+    if what == "tau":
+        c.tau = 0.5
+    elif what == "mach":
+        c.inlet_ux = 0.2
+    elif what == "periodic":
+        c.top = Wall(WallKind.periodic)
+    elif what == "inlet":
+        m.solid[16 * 3] = 1
+    assert name == "lattice2d-" + what
+    assert message == refcheck.validate2(c, m) and message
+
+
+def test_2d_lattice_mask_shape_is_checked_before_the_pointer_crosses(driver):
+    out = subprocess.run([driver, "lattice2d-size"], check=True, capture_output=True, text=True).stdout.strip()
+    assert out == "lattice2d-size\tlbm2d: the solid mask does not match the lattice size"
