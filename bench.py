@@ -433,6 +433,124 @@ def serve_bench(args):
     return 0
 
 
+# The reference's published numbers for its own GPU benchmarks (AMD Radeon RX 9060 XT, OpenGL
+# compute, binary32 shifted populations, A-B ping-pong): docs/performance/
+# results-windows-msvc-release.json:370-420 and docs/lbm.md:536-538.
+REFERENCE_GPU = {
+    "d3q19_128_bgk_steps_s": 681.9, "d3q19_128_les_steps_s": 676.7,
+    "d2q9_2048x1024_bgk_steps_s": 1521.0, "d2q9_2048x1024_les_steps_s": 1475.0,
+    "adapter_sphere_re100_s": 16.4, "adapter_sphere_re1000_s": 202.0,
+}
+
+
+def reference_sphere_channel(les):
+    """sphere_channel(128, 128, 128, 24, 2.5, 0.05, 300) of tests/aero/lbm3d/lbm3d_test_support.h:
+    248-270 (a quarter node off the axis, sphere_mask), Smagorinsky {les, 0.1}: the setup of the
+    reference's D3Q19 GPU benchmark (tests/bench/aero/bench_lbm_gpu.cpp:89-108)."""
+    from orrery_amd import Lbm3DConfig, Smagorinsky, Wall, WallKind, XBoundary, sphere_mask, tau_of_viscosity
+    n, d, u, re = 128, 24.0, 0.05, 300.0
+    config = Lbm3DConfig(nx=n, ny=n, nz=n, tau=tau_of_viscosity(u * d / re), x_boundary=XBoundary.inflow_outflow,
+                         y_low=Wall(WallKind.slip), y_high=Wall(WallKind.slip), z_low=Wall(WallKind.slip),
+                         z_high=Wall(WallKind.slip), inlet_ux=u, smagorinsky=Smagorinsky(les, 0.1))
+    centre = (2.5 * d, 0.5 * (n - 1) + 0.25, 0.5 * (n - 1) + 0.25)
+    return config, sphere_mask(n, n, n, centre, 0.5 * d)
+
+
+def reference_cylinder_channel(les):
+    """cylinder_setup(2048, 1024, 64, 400, 512.5, 0.05, 1000) of tests/aero/lbm/lbm_test_support.h:
+    154-167 (convective outlet, slip sides), Smagorinsky {les, 0.1}: the setup of the reference's
+    D2Q9 GPU benchmark (bench_lbm_gpu.cpp:111-133)."""
+    from orrery_amd import Lbm2DConfig, Smagorinsky, Wall, WallKind, XBoundary, circle_mask, tau_of_viscosity
+    d, u, re = 64.0, 0.05, 1000.0
+    config = Lbm2DConfig(nx=2048, ny=1024, tau=tau_of_viscosity(u * d / re), x_boundary=XBoundary.inflow_outflow,
+                         bottom=Wall(WallKind.slip), top=Wall(WallKind.slip), inlet_ux=u,
+                         smagorinsky=Smagorinsky(les, 0.1))
+    return config, circle_mask(2048, 1024, 400.0, 512.5, 0.5 * d)
+
+
+def steps_per_second(solver, count=1000, warmup=50):
+    """The reference's method (bench_lbm_gpu.cpp:53-65): warm-up steps, the device drained, then
+    steady-clock time around `count` steps and a drain."""
+    solver.step(warmup)
+    solver.synchronize()
+    t0 = time.perf_counter()
+    solver.step(count)
+    solver.synchronize()
+    return count / (time.perf_counter() - t0)
+
+
+def adapter_sphere_solve(reynolds, cells, upstream, downstream, width, duration, average):
+    """One adapter solve of the reference's sphere validation (tests/aero_validation/lbm3d/
+    test_sphere.cpp, body_case.h): setup (voxelisation included) and every step, wall clock."""
+    import math
+    from orrery_amd import geometry
+    from orrery_amd.aero import (AeroGeometry, FlowConditions, LbmAeroSolver3D, LbmSolver3DOptions, LesMode,
+                                 ReferenceValues)
+    geo = AeroGeometry(geometry.sphere_mesh(0.5, geometry.RevolutionResolution(64, 0.05)),
+                       ReferenceValues(area=0.25 * math.pi, length=1.0))
+    cond = FlowConditions(speed=0.1, density=1000.0)
+    cond.viscosity = cond.density * cond.speed / reynolds
+    opt = LbmSolver3DOptions(cells_per_length=cells, upstream=upstream, downstream=downstream, width=width,
+                             duration=duration, average_window=average, les=LesMode.off, steps_per_call=2000)
+    t0 = time.perf_counter()
+    s = LbmAeroSolver3D(opt)
+    s.setup(geo, cond)
+    r = s.run()
+    wall = time.perf_counter() - t0
+    c = s.info().config
+    s.close()
+    return {"seconds": wall, "cd": r.cd, "iterations": int(r.iterations), "grid": f"{c.nx}x{c.ny}x{c.nz}",
+            "schiller_naumann": 24.0 / reynolds * (1.0 + 0.15 * reynolds ** 0.687)}
+
+
+def reference_bench(args):
+    """The reference's own GPU benchmarks and adapter solves, run here, beside its published
+    numbers on its GPU: D3Q19 128^3 sphere channel and D2Q9 2048 x 1024 cylinder (BGK and
+    Smagorinsky, steps per second over 1000 steps), and the Re 100 / Re 1000 sphere solves of its
+    validation (seconds per solve)."""
+    from orrery_amd import Lbm2DSolver, Lbm3DSolver
+    out = {}
+    for les in (False, True):
+        tag = "les" if les else "bgk"
+        print(f"bench: reference D3Q19 128^3 sphere channel ({tag})", file=sys.stderr, flush=True)
+        config, mask = reference_sphere_channel(les)
+        s = Lbm3DSolver(config, mask, "f32")
+        rate = steps_per_second(s)
+        out[f"d3q19_128_{tag}"] = {"steps_s": rate, "mlups": rate * config.cells / 1e6,
+                                   "reference_steps_s": REFERENCE_GPU[f"d3q19_128_{tag}_steps_s"],
+                                   "ratio": rate / REFERENCE_GPU[f"d3q19_128_{tag}_steps_s"]}
+        s.close()
+        print(f"bench: reference D2Q9 2048x1024 cylinder ({tag})", file=sys.stderr, flush=True)
+        config2, mask2 = reference_cylinder_channel(les)
+        s2 = Lbm2DSolver(config2, mask2, "f32")
+        rate2 = steps_per_second(s2)
+        out[f"d2q9_2048x1024_{tag}"] = {"steps_s": rate2, "mlups": rate2 * config2.cells / 1e6,
+                                        "reference_steps_s": REFERENCE_GPU[f"d2q9_2048x1024_{tag}_steps_s"],
+                                        "ratio": rate2 / REFERENCE_GPU[f"d2q9_2048x1024_{tag}_steps_s"]}
+        s2.close()
+    for re, case in ((100.0, (16.0, 4.0, 8.0, 6.0, 30.0, 10.0)), (1000.0, (48.0, 2.0, 4.0, 4.0, 20.0, 10.0))):
+        print(f"bench: reference adapter sphere solve, Re {re:g}", file=sys.stderr, flush=True)
+        r = adapter_sphere_solve(re, *case)
+        key = f"adapter_sphere_re{int(re)}"
+        r["reference_seconds"] = REFERENCE_GPU[f"{key}_s"]
+        r["ratio"] = REFERENCE_GPU[f"{key}_s"] / r["seconds"]
+        out[key] = r
+    head = out["d3q19_128_bgk"]
+    line = {
+        "metric": "the reference's GPU benchmark: D3Q19 128^3 sphere channel steps/s (D 24, Re 300, BGK)",
+        "value": round(head["steps_s"], 1), "unit": "steps/s", "n_gpus": 1, "steps": 1000, "warmup": 50,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": round(head["ratio"], 2), "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "tests/bench/aero/bench_lbm_gpu.cpp cases and the sphere validation solves of "
+                               "tests/aero_validation/lbm3d, at the reference's sizes", "parallelism": "single GPU"},
+        "baseline_hardware": "AMD Radeon RX 9060 XT (the reference's published runs)",
+        "cases": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                  for k, v in out.items()},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -445,10 +563,12 @@ def main(argv=None):
     ap.add_argument("--compare-steps", type=int, default=2000,
                     help="step at which every storage's force and velocity are compared with the f64 storage's "
                          "(0: no comparison, no f64 run)")
-    ap.add_argument("--workload", choices=["channel", "polar", "serve", "d2q9", "sphere"], default="channel",
+    ap.add_argument("--workload", choices=["channel", "polar", "serve", "d2q9", "sphere", "reference"],
+                    default="channel",
                     help="channel: the 512x256x256 lattice (the headline line); polar: the batched wing-section "
                          "sweep, batched against one at a time; serve: the continuous-batching server; d2q9: the "
-                         "400x160 cylinder channel; sphere: the 256x128x128 sphere channel")
+                         "400x160 cylinder channel; sphere: the 256x128x128 sphere channel; reference: the "
+                         "reference's own GPU benchmarks and adapter solves beside its published numbers")
     ap.add_argument("--d2q9-steps", type=int, default=20000, help="d2q9: configured steps (the run is twice as long)")
     ap.add_argument("--sphere-steps", type=int, default=5000, help="sphere: steps of the sampled runs")
     ap.add_argument("--lattices", type=int, default=48, help="polar: number of wing cases")
@@ -477,6 +597,8 @@ def main(argv=None):
         return d2q9_bench(args)
     if args.workload == "sphere":
         return sphere_bench(args)
+    if args.workload == "reference":
+        return reference_bench(args)
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
     config, mask = workload()
