@@ -50,7 +50,8 @@ def workload(nx=512, ny=256, nz=256, diameter=64.0, pitch_deg=20.0, reynolds=100
     return config, place_body(mesh, nx, ny, nz, interpolated=False, device=True)
 
 
-def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare_steps=0, ref=None, tune_steps=0):
+def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare_steps=0, ref=None, tune_steps=0,
+                 groups=(0, 0)):
     """Times `steps` steps after `warmup` (one event pair) and `per_kernel_steps` more per kernel,
     then steps on to `compare_steps` and reads the force and velocity there: returned as the
     reference fields for the f64 storage, compared with `ref` (the f64 run's) otherwise. With
@@ -68,13 +69,16 @@ def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare
     if tune_steps > 0:
         best, tried = autotune.tune(solver, tune_steps)
         launch = {"schedule": best.schedule, "block": best.block, "cells_per_thread": best.cells_per_thread,
+                  "groups_per_cu": [best.groups_a, best.groups_b],
                   "tuned_ms_per_step": round(best.ms_per_step, 4),
                   "default_ms_per_step": round(next(t.ms_per_step for t in tried if t.schedule == "serial" and t.block == 256
-                                                    and t.cells_per_thread == (2 if storage in ("f16s", "bf16s") else 1)), 4),
+                                                    and t.cells_per_thread == (2 if storage in ("f16s", "bf16s") else 1)
+                                                    and t.groups_a == 0 and t.groups_b == 0), 4),
                   "candidates": len(tried)}
     else:
+        solver.set_occupancy(*groups)
         sched, block, cpt = autotune.current(solver)
-        launch = {"schedule": sched, "block": block, "cells_per_thread": cpt}
+        launch = {"schedule": sched, "block": block, "cells_per_thread": cpt, "groups_per_cu": list(solver.occupancy())}
     ms = ctypes.c_double()
     bulk = ctypes.c_double()
     bnd = ctypes.c_double()
@@ -558,8 +562,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--storages", default="f32,f16s,bf16s")
-    ap.add_argument("--tune-steps", type=int, default=10,
+    ap.add_argument("--tune-steps", type=int, default=30,
                     help="steps per candidate of the launch autotuner before timing (0: the default launch)")
+    ap.add_argument("--groups-per-cu", default="0,0", metavar="A,B",
+                    help="with --tune-steps 0: occupancy caps of the bulk kernel, A step and B step (0: none)")
     ap.add_argument("--compare-steps", type=int, default=2000,
                     help="step at which every storage's force and velocity are compared with the f64 storage's "
                          "(0: no comparison, no f64 run)")
@@ -614,7 +620,8 @@ def main(argv=None):
         print(f"bench: timing {storage} ({args.warmup} warm-up, {args.steps} timed steps"
               + (f", compared with f64 at step {compare})" if compare else ")"), file=sys.stderr, flush=True)
         results[storage] = time_storage(config, mask, storage, args.steps, args.warmup, per_kernel_steps, compare,
-                                        ref, args.tune_steps)
+                                        ref, args.tune_steps,
+                                        groups=tuple(int(g) for g in args.groups_per_cu.split(",")))
         if storage == "f64" and compare:
             ref = {"force_x": results[storage]["force_x"], "fields": results[storage].pop("fields")}
     ref = None

@@ -165,16 +165,28 @@ int lbm3d_set_schedule(lbm3d* h, int32_t schedule);
 int lbm3d_set_graphs(lbm3d* h, int32_t enabled);
 /* The schedule and bulk launch shape lbm3d_step uses. */
 int lbm3d_get_launch(const lbm3d* h, int32_t* schedule, int32_t* block, int32_t* cells_per_thread);
+/* Cap on the workgroups of the bulk kernel that share a compute unit, for the A step (odd step
+ * counts) and the B step (even), 1 to 32, 0 for none (the default): each workgroup reserves LDS
+ * it does not use, so that no more than that many fit (with 256-thread workgroups, the cap is
+ * waves per SIMD). Fewer resident waves stream the 19 population planes with fewer DRAM
+ * conflicts; results do not change. Refused when the device's LDS cannot express the cap. No
+ * reference counterpart; the autotuner sets it. */
+int lbm3d_set_occupancy(lbm3d* h, int32_t groups_a, int32_t groups_b);
+int lbm3d_get_occupancy(const lbm3d* h, int32_t* groups_a, int32_t* groups_b);
 /* One candidate of the autotuner: a schedule with a bulk launch shape, and its measured time. */
 typedef struct lbm_launch_choice {
     int32_t schedule;          /* LBM_SCHEDULE_* */
     int32_t block;             /* threads per block of the bulk kernel */
     int32_t cells_per_thread;  /* consecutive cells per thread of the bulk kernel */
+    int32_t groups_a, groups_b; /* occupancy caps of lbm3d_set_occupancy (0: none) */
     double ms_per_step;
 } lbm_launch_choice;
 /* Autotuner (no reference counterpart): times `steps` steps (rounded up to even) under every
  * bulk launch shape (block 64/128/192/256 x 1/2/4 cells per thread) of the serial schedule, and
- * the fused schedule unless serial_only, then keeps the fastest on the handle. The lattice is
+ * the fused schedule unless serial_only; under the fastest serial shape it then tries caps of
+ * 5, 4 and 3 workgroups per compute unit on the A step, then on the B step, then on the A step
+ * again (lbm3d_set_occupancy); the four fastest are timed once more, and the fastest of that
+ * final round is kept on the handle. The lattice is
  * copied on the device first and restored afterwards, with its step count: its later results
  * are those it would have had without tuning, bit for bit (all schedules and shapes compute
  * the same values). Needs device memory for that copy (LBM_ENOMEM otherwise). `tried` (capacity

@@ -58,7 +58,7 @@ class COscillation(ctypes.Structure):
 
 class CLaunchChoice(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int32), ("block", ctypes.c_int32), ("cells_per_thread", ctypes.c_int32),
-                ("ms_per_step", ctypes.c_double)]
+                ("groups_a", ctypes.c_int32), ("groups_b", ctypes.c_int32), ("ms_per_step", ctypes.c_double)]
 
 
 D = ctypes.c_double
@@ -147,6 +147,8 @@ EXPORTS = [
     ("lbm3d_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, P(CLaunchChoice),
                                       ctypes.c_size_t, c_sizep, P(CLaunchChoice)]),
     ("lbm3d_set_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    ("lbm3d_set_occupancy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    ("lbm3d_get_occupancy", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int32), P(ctypes.c_int32)]),
     ("lbm3d_step_bytes", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_counts", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep, c_sizep, c_sizep]),
     ("lbm3d_time_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, c_f64p, c_f64p, c_f64p]),

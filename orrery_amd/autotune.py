@@ -1,5 +1,6 @@
-"""Autotuner of the step launch: which schedule and bulk launch shape (threads per block, cells
-per thread) run a lattice fastest on this device.
+"""Autotuner of the step launch: which schedule, bulk launch shape (threads per block, cells
+per thread) and bulk occupancy caps (workgroups per compute unit, A and B step) run a lattice
+fastest on this device.
 
 The library does the timing (lbm3d_autotune: every shape on a device copy of the lattice, which
 is restored afterwards, so tuning changes no result). This module adds a JSON cache keyed by the
@@ -29,11 +30,13 @@ class LaunchChoice:
     block: int
     cells_per_thread: int
     ms_per_step: float
+    groups_a: int = 0  # occupancy caps (workgroups per compute unit), 0: none
+    groups_b: int = 0
 
     @staticmethod
     def _from_c(c: abi.CLaunchChoice) -> "LaunchChoice":
         return LaunchChoice(_SCHEDULE_NAMES.get(int(c.schedule), str(c.schedule)), int(c.block),
-                            int(c.cells_per_thread), float(c.ms_per_step))
+                            int(c.cells_per_thread), float(c.ms_per_step), int(c.groups_a), int(c.groups_b))
 
 
 def device_name() -> str:
@@ -81,6 +84,7 @@ def current(solver: Lbm3DSolver) -> tuple[str, int, int]:
 def apply(solver: Lbm3DSolver, choice: LaunchChoice) -> None:
     solver.set_schedule(choice.schedule)
     solver.set_launch(choice.block, choice.cells_per_thread)
+    solver.set_occupancy(choice.groups_a, choice.groups_b)
 
 
 def tune(solver: Lbm3DSolver, steps: int = 10, serial_only: bool = False) -> tuple[LaunchChoice, list[LaunchChoice]]:
@@ -104,7 +108,8 @@ def autotune(solver: Lbm3DSolver, steps: int = 10, cache: str | None = None, ref
     data = _load_cache(path)
     if not refresh and key in data:
         e = data[key]
-        choice = LaunchChoice(e["schedule"], int(e["block"]), int(e["cells_per_thread"]), float(e["ms_per_step"]))
+        choice = LaunchChoice(e["schedule"], int(e["block"]), int(e["cells_per_thread"]), float(e["ms_per_step"]),
+                              int(e.get("groups_a", 0)), int(e.get("groups_b", 0)))
         apply(solver, choice)
         return choice
     choice, _ = tune(solver, steps)

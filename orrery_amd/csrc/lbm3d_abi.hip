@@ -93,6 +93,17 @@ void release(lbm3d* h) {
     delete h;
 }
 
+// LDS each bulk workgroup reserves for the handle's cap on workgroups per compute unit (A step
+// when `local`), rounded down to 512-byte allocation granules: `groups` fit in a compute unit's
+// LDS, `groups + 1` do not. 0 without a cap.
+std::uint32_t bulk_lds(const lbm3d& h, bool local) {
+    const int groups = h.groups[local ? 0 : 1];
+    if (groups <= 0 || h.lds_per_cu == 0) {
+        return 0;
+    }
+    return h.lds_per_cu / static_cast<std::uint32_t>(groups) / 512U * 512U;
+}
+
 // One step under the handle's schedule: one fused launch; the bulk and boundary kernels one
 // after the other; or the two kernels concurrently on two streams.
 template <class T> hipError_t step_once(lbm3d& h, const lbm::Lattice3<typename T::R>& p, bool local) {
@@ -100,12 +111,12 @@ template <class T> hipError_t step_once(lbm3d& h, const lbm::Lattice3<typename T
         return lbm::d3q19_step_fused<T>(p, local, h.stream);
     }
     if (h.schedule == LBM_SCHEDULE_SERIAL || p.nb == 0) {
-        return lbm::d3q19_step<T>(p, local, h.block, h.cpt, h.stream);
+        return lbm::d3q19_step<T>(p, local, h.block, h.cpt, bulk_lds(h, local), h.stream);
     }
     hipError_t e = hipEventRecord(h.fork, h.stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(h.side, h.fork, 0);
     if (e == hipSuccess) e = lbm::d3q19_step_boundary<T>(p, local, h.side);
-    if (e == hipSuccess) e = lbm::d3q19_step_bulk<T>(p, local, h.block, h.cpt, h.stream);
+    if (e == hipSuccess) e = lbm::d3q19_step_bulk<T>(p, local, h.block, h.cpt, bulk_lds(h, local), h.stream);
     if (e == hipSuccess) e = hipEventRecord(h.join, h.side);
     if (e == hipSuccess) e = hipStreamWaitEvent(h.stream, h.join, 0);
     return e;
@@ -214,6 +225,16 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
                                     " are available; use fewer cells");
     }
     hipError_t e = hipGetDevice(&h->device);
+    int lds_cu = 0;
+    int lds_block = 0;
+    if (e == hipSuccess) {
+        e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, h->device);
+    }
+    if (e == hipSuccess) {
+        e = hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
+    }
+    h->lds_per_cu = static_cast<std::uint32_t>(lds_cu > 0 ? lds_cu : 0);
+    h->lds_per_block = static_cast<std::uint32_t>(lds_block > 0 ? lds_block : 0);
     if (e == hipSuccess) {
         e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     }
@@ -358,6 +379,7 @@ int lbm3d_step(lbm3d* h, uint64_t count) {
         lbm::key_append(key, h->schedule);
         lbm::key_append(key, h->block);
         lbm::key_append(key, h->cpt);
+        lbm::key_append(key, h->groups);
         const hipError_t e = h->graph.run(h->stream, count, h->steps, key, launch);
         return e == hipSuccess ? ok() : hip_fail("step", e);
     });
@@ -689,7 +711,7 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
                     const hipError_t err = step_once<T>(*h, p, local);
                     return err == hipSuccess ? ok() : hip_fail("timed step", err);
                 }
-                hipError_t err = lbm::d3q19_step_bulk<T>(p, local, h->block, h->cpt, h->stream);
+                hipError_t err = lbm::d3q19_step_bulk<T>(p, local, h->block, h->cpt, bulk_lds(*h, local), h->stream);
                 if (err == hipSuccess && per_kernel) {
                     err = hipEventRecord(ev[3U * n + 1U], h->stream);
                 }
@@ -743,6 +765,34 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
     return rc;
 }
 
+int lbm3d_set_occupancy(lbm3d* h, int32_t groups_a, int32_t groups_b) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    for (const int32_t g : {groups_a, groups_b}) {
+        if (g < 0 || g > 32) {
+            return fail(LBM_EINVAL, "lbm3d: workgroups per compute unit must be 0 (no cap) to 32");
+        }
+        if (g > 0 && (h->lds_per_cu == 0 || h->lds_per_cu / static_cast<std::uint32_t>(g) / 512U * 512U >
+                                                   h->lds_per_block)) {
+            return fail(LBM_EINVAL, "lbm3d: the device's LDS cannot express a cap of " + std::to_string(g) +
+                                        " workgroups per compute unit");
+        }
+    }
+    h->groups[0] = groups_a;
+    h->groups[1] = groups_b;
+    return ok();
+}
+
+int lbm3d_get_occupancy(const lbm3d* h, int32_t* groups_a, int32_t* groups_b) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    if (groups_a != nullptr) *groups_a = h->groups[0];
+    if (groups_b != nullptr) *groups_b = h->groups[1];
+    return ok();
+}
+
 int lbm3d_get_launch(const lbm3d* h, int32_t* schedule, int32_t* block, int32_t* cells_per_thread) {
     if (h == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null handle");
@@ -788,30 +838,98 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
     const int saved_schedule = h->schedule;
     const int saved_block = h->block;
     const int saved_cpt = h->cpt;
+    const int saved_groups[2] = {h->groups[0], h->groups[1]};
     const std::uint64_t saved_steps = h->steps;
     std::vector<lbm_launch_choice> cand;
     for (const int block : {256, 192, 128, 64}) {
         for (const int cpt : {1, 2, 4}) {
-            cand.push_back({LBM_SCHEDULE_SERIAL, block, cpt, 0.0});
+            cand.push_back({LBM_SCHEDULE_SERIAL, block, cpt, 0, 0, 0.0});
         }
     }
     if (!serial_only) {
-        cand.push_back({LBM_SCHEDULE_FUSED, 256, 1, 0.0});
+        cand.push_back({LBM_SCHEDULE_FUSED, 256, 1, 0, 0, 0.0});
     }
     int rc = LBM_OK;
-    for (auto& c : cand) {
+    const auto time_candidate = [&](lbm_launch_choice& c) {
         h->schedule = c.schedule;
         h->block = c.block;
         h->cpt = c.cells_per_thread;
+        h->groups[0] = c.groups_a;
+        h->groups[1] = c.groups_b;
         double ms = 0.0;
         rc = lbm3d_time_steps(h, 2, 0, &ms, nullptr, nullptr); // warm-up
         if (rc == LBM_OK) {
             rc = lbm3d_time_steps(h, timed, 0, &ms, nullptr, nullptr);
         }
-        if (rc != LBM_OK) {
+        c.ms_per_step = ms / static_cast<double>(timed);
+        return rc == LBM_OK;
+    };
+    std::size_t final_begin = 0; // candidates before it take no part in the choice
+    const auto fastest = [&]() {
+        std::size_t k = final_begin;
+        for (std::size_t i = final_begin + 1; i < cand.size(); ++i) {
+            if (cand[i].ms_per_step < cand[k].ms_per_step) {
+                k = i;
+            }
+        }
+        return k;
+    };
+    for (auto& c : cand) {
+        if (!time_candidate(c)) {
             break;
         }
-        c.ms_per_step = ms / static_cast<double>(timed);
+    }
+    // Occupancy caps for the bulk kernel of the fastest serial shape: fewer resident
+    // workgroups per compute unit stream the 19 planes with fewer conflicts. The A-step cap,
+    // then the B-step cap under the best A-step cap, then the A-step cap again under that.
+    if (rc == LBM_OK && cand[fastest()].schedule == LBM_SCHEDULE_SERIAL && h->lds_per_cu > 0) {
+        for (int pass = 0; pass < 3 && rc == LBM_OK; ++pass) {
+            const int side = pass & 1;
+            const lbm_launch_choice base = cand[fastest()];
+            for (const int g : {5, 4, 3}) {
+                lbm_launch_choice c = base;
+                (side == 0 ? c.groups_a : c.groups_b) = g;
+                if (h->lds_per_cu / static_cast<std::uint32_t>(g) / 512U * 512U > h->lds_per_block) {
+                    continue;
+                }
+                if (!time_candidate(c)) {
+                    break;
+                }
+                cand.push_back(c);
+            }
+        }
+    }
+    // The fastest few, timed once more in turn: the first timings of candidates a percent apart
+    // are within each other's noise, and the fastest of many noisy timings is biased low.
+    if (rc == LBM_OK && cand.size() > 1) {
+        std::vector<std::size_t> order(cand.size());
+        for (std::size_t i = 0; i < order.size(); ++i) {
+            order[i] = i;
+        }
+        std::sort(order.begin(), order.end(),
+                  [&](std::size_t a, std::size_t b) { return cand[a].ms_per_step < cand[b].ms_per_step; });
+        std::vector<lbm_launch_choice> finals;
+        for (std::size_t i = 0; i < order.size() && finals.size() < 4; ++i) {
+            const lbm_launch_choice& c = cand[order[i]];
+            bool seen = false;
+            for (const auto& f : finals) {
+                seen = seen || (f.schedule == c.schedule && f.block == c.block &&
+                                f.cells_per_thread == c.cells_per_thread && f.groups_a == c.groups_a &&
+                                f.groups_b == c.groups_b);
+            }
+            if (!seen) {
+                finals.push_back(c);
+            }
+        }
+        for (auto& c : finals) {
+            if (!time_candidate(c)) {
+                break;
+            }
+            cand.push_back(c);
+        }
+        if (rc == LBM_OK) {
+            final_begin = cand.size() - finals.size(); // only the final round competes
+        }
     }
     e = hipMemcpyAsync(f_base, s, f_bytes, hipMemcpyDeviceToDevice, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(h->stash, s + f_bytes, stash_bytes, hipMemcpyDeviceToDevice, h->stream);
@@ -827,17 +945,16 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
         h->schedule = saved_schedule;
         h->block = saved_block;
         h->cpt = saved_cpt;
+        h->groups[0] = saved_groups[0];
+        h->groups[1] = saved_groups[1];
         return rc;
     }
-    std::size_t k = 0;
-    for (std::size_t i = 1; i < cand.size(); ++i) {
-        if (cand[i].ms_per_step < cand[k].ms_per_step) {
-            k = i;
-        }
-    }
+    const std::size_t k = fastest();
     h->schedule = cand[k].schedule;
     h->block = cand[k].block;
     h->cpt = cand[k].cells_per_thread;
+    h->groups[0] = cand[k].groups_a;
+    h->groups[1] = cand[k].groups_b;
     if (best != nullptr) *best = cand[k];
     if (tried != nullptr) {
         for (std::size_t i = 0; i < cand.size() && i < capacity; ++i) {

@@ -539,19 +539,21 @@ inline std::uint32_t blocks_for(std::uint64_t n, std::uint32_t block) {
     return static_cast<std::uint32_t>((n + block - 1) / block);
 }
 
+// `lds`: bytes of LDS each workgroup reserves without using them, which caps how many
+// workgroups share a compute unit (0: no cap).
 template <class T, bool kLocal>
-hipError_t step_bulk(const Lattice3<typename T::R>& p, int block, int cpt, hipStream_t s) {
+hipError_t step_bulk(const Lattice3<typename T::R>& p, int block, int cpt, std::uint32_t lds, hipStream_t s) {
     const std::uint32_t per = static_cast<std::uint32_t>(block) * static_cast<std::uint32_t>(cpt);
     const std::uint32_t grid = blocks_for(p.cells, per);
     switch (cpt) {
         case 2:
-            hipLaunchKernelGGL((d3q19_bulk<T, kLocal, 2>), dim3(grid), dim3(block), 0, s, p);
+            hipLaunchKernelGGL((d3q19_bulk<T, kLocal, 2>), dim3(grid), dim3(block), lds, s, p);
             break;
         case 4:
-            hipLaunchKernelGGL((d3q19_bulk<T, kLocal, 4>), dim3(grid), dim3(block), 0, s, p);
+            hipLaunchKernelGGL((d3q19_bulk<T, kLocal, 4>), dim3(grid), dim3(block), lds, s, p);
             break;
         default:
-            hipLaunchKernelGGL((d3q19_bulk<T, kLocal, 1>), dim3(grid), dim3(block), 0, s, p);
+            hipLaunchKernelGGL((d3q19_bulk<T, kLocal, 1>), dim3(grid), dim3(block), lds, s, p);
             break;
     }
     return hipGetLastError();
@@ -559,8 +561,9 @@ hipError_t step_bulk(const Lattice3<typename T::R>& p, int block, int cpt, hipSt
 } // namespace
 
 template <class T>
-hipError_t d3q19_step_bulk(const Lattice3<typename T::R>& p, bool local, int block, int cpt, hipStream_t s) {
-    return local ? step_bulk<T, true>(p, block, cpt, s) : step_bulk<T, false>(p, block, cpt, s);
+hipError_t d3q19_step_bulk(const Lattice3<typename T::R>& p, bool local, int block, int cpt, std::uint32_t lds,
+                           hipStream_t s) {
+    return local ? step_bulk<T, true>(p, block, cpt, lds, s) : step_bulk<T, false>(p, block, cpt, lds, s);
 }
 
 template <class T>
@@ -588,8 +591,9 @@ template <class T> hipError_t d3q19_step_fused(const Lattice3<typename T::R>& p,
 }
 
 template <class T>
-hipError_t d3q19_step(const Lattice3<typename T::R>& p, bool local, int block, int cpt, hipStream_t s) {
-    const hipError_t e = d3q19_step_bulk<T>(p, local, block, cpt, s);
+hipError_t d3q19_step(const Lattice3<typename T::R>& p, bool local, int block, int cpt, std::uint32_t lds,
+                      hipStream_t s) {
+    const hipError_t e = d3q19_step_bulk<T>(p, local, block, cpt, lds, s);
     if (e != hipSuccess) {
         return e;
     }
@@ -683,8 +687,8 @@ hipError_t d3q19_force_sum(const Lattice3<typename T::R>& p, int phase, const st
 }
 
 #define LBM_INSTANTIATE3(T)                                                                                  \
-    template hipError_t d3q19_step<T>(const Lattice3<T::R>&, bool, int, int, hipStream_t);                   \
-    template hipError_t d3q19_step_bulk<T>(const Lattice3<T::R>&, bool, int, int, hipStream_t);              \
+    template hipError_t d3q19_step<T>(const Lattice3<T::R>&, bool, int, int, std::uint32_t, hipStream_t);    \
+    template hipError_t d3q19_step_bulk<T>(const Lattice3<T::R>&, bool, int, int, std::uint32_t, hipStream_t); \
     template hipError_t d3q19_step_boundary<T>(const Lattice3<T::R>&, bool, hipStream_t);                    \
     template hipError_t d3q19_step_fused<T>(const Lattice3<T::R>&, bool, hipStream_t);                       \
     template hipError_t d3q19_initialize<T>(const Lattice3<T::R>&, int, const double*, const double*,       \

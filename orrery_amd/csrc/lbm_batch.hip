@@ -697,7 +697,7 @@ int lbm_batch_autotune(lbm_batch* b, uint32_t steps, lbm_launch_choice* best) {
         }
     }
     for (int st = 0; st < 4; ++st) {
-        best[st] = lbm_launch_choice{LBM_SCHEDULE_SERIAL, 0, 0, 0.0};
+        best[st] = lbm_launch_choice{LBM_SCHEDULE_SERIAL, 0, 0, 0, 0, 0.0};
         // The largest lattice of this storage stands for the group, copied into a scratch
         // handle (halfway walls: the bulk kernel, which the shape tunes, is the same).
         const lbm_batch::Entry* pick = nullptr;

@@ -38,6 +38,11 @@ struct lbm3d {
     double inlet[3] = {0, 0, 0};
     int block = 256;
     int cpt = 1;
+    // Cap on the bulk kernel's workgroups per compute unit, A step and B step (0: none), and
+    // the device's LDS per compute unit and per workgroup that the cap is reserved from.
+    int groups[2] = {0, 0};
+    std::uint32_t lds_per_cu = 0;
+    std::uint32_t lds_per_block = 0;
     // device memory
     void* f = nullptr;
     std::uint8_t* cls = nullptr;

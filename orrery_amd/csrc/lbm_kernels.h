@@ -13,10 +13,12 @@ namespace lbm {
 // One lattice step: bulk kernel over all cells, boundary kernel over the boundary list.
 // `local` is the A step (odd step count), otherwise the B step.
 template <class T>
-hipError_t d3q19_step(const Lattice3<typename T::R>& p, bool local, int block, int cpt, hipStream_t s);
+hipError_t d3q19_step(const Lattice3<typename T::R>& p, bool local, int block, int cpt, std::uint32_t lds,
+                      hipStream_t s);
 // The two halves of a step, for per-kernel timing.
 template <class T>
-hipError_t d3q19_step_bulk(const Lattice3<typename T::R>& p, bool local, int block, int cpt, hipStream_t s);
+hipError_t d3q19_step_bulk(const Lattice3<typename T::R>& p, bool local, int block, int cpt, std::uint32_t lds,
+                           hipStream_t s);
 template <class T>
 hipError_t d3q19_step_boundary(const Lattice3<typename T::R>& p, bool local, hipStream_t s);
 // The whole step in one launch (bulk and boundary nodes in cell order).

@@ -219,6 +219,16 @@ class Lbm3DSolver:
     def set_launch(self, block: int = 0, cells_per_thread: int = 0) -> None:
         _check(self._lib, self._lib.lbm3d_set_launch(self._h, block, cells_per_thread))
 
+    def set_occupancy(self, groups_a: int, groups_b: int) -> None:
+        """Cap on the bulk kernel's workgroups per compute unit for the A step and the B step (0:
+        none); results do not change."""
+        _check(self._lib, self._lib.lbm3d_set_occupancy(self._h, int(groups_a), int(groups_b)))
+
+    def occupancy(self) -> tuple[int, int]:
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _check(self._lib, self._lib.lbm3d_get_occupancy(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def set_schedule(self, schedule: str) -> None:
         """"serial" (bulk then boundary kernel, the default), "fused" (one kernel per step) or
         "streams" (the two kernels concurrently on two streams); all give identical results."""

@@ -205,8 +205,8 @@ def test_call_size_independence_is_bitwise():
 @pytest.mark.parametrize("storage,x_periodic", [("f32", False), ("f16s", False), ("f64", False), ("f32", True),
                                                 ("f16s", True)])
 def test_schedules_and_launch_shapes_are_bitwise_identical(storage, x_periodic):
-    """One fused kernel per step, the two kernels in turn (under every bulk launch shape), and
-    the two kernels on two streams give the same state bit for bit. 70 x 21 x 19 cells is not a
+    """One fused kernel per step, the two kernels in turn (under every bulk launch shape and
+    occupancy cap), and the two kernels on two streams give the same state bit for bit. 70 x 21 x 19 cells is not a
     multiple of 64, so the last wave of the fused kernel is partial; with 4 cells per thread
     some runs hold the x = nx - 1 node of one row and the x = 0 node of the next. With periodic
     x, the x-face nodes pull across the row ends."""
@@ -217,17 +217,20 @@ def test_schedules_and_launch_shapes_are_bitwise_identical(storage, x_periodic):
         c.x_boundary = XBoundary.periodic
         c.body_force_x = 1e-6
     runs = []
-    for schedule, block, cpt in [("fused", 0, 0), ("serial", 256, 1), ("serial", 128, 2), ("serial", 64, 4),
-                                 ("streams", 192, 1)]:
+    for schedule, block, cpt, caps in [("fused", 0, 0, (0, 0)), ("serial", 256, 1, (0, 0)), ("serial", 128, 2, (0, 0)),
+                                       ("serial", 64, 4, (0, 0)), ("streams", 192, 1, (0, 0)),
+                                       ("serial", 256, 1, (4, 3)), ("streams", 256, 2, (2, 5))]:
         s = Lbm3DSolver(c, mask, storage)
         s.set_schedule(schedule)
         s.set_launch(block, cpt)
+        s.set_occupancy(*caps)
+        assert s.occupancy() == caps
         s.initialize_perturbed(5, 1e-4)
         s.step(41)
-        runs.append((schedule, block, cpt, s.populations(), s.force()))
-    for schedule, block, cpt, f, force in runs[1:]:
-        print(f"{storage}: {schedule} block {block} cpt {cpt}", flush=True)
-        assert np.array_equal(f, runs[0][3]) and force == runs[0][4]
+        runs.append((schedule, block, cpt, caps, s.populations(), s.force()))
+    for schedule, block, cpt, caps, f, force in runs[1:]:
+        print(f"{storage}: {schedule} block {block} cpt {cpt} workgroups per CU {caps}", flush=True)
+        assert np.array_equal(f, runs[0][4]) and force == runs[0][5]
 
 
 def test_ramped_inlet_agrees():
