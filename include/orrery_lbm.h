@@ -262,10 +262,16 @@ int lbm_batch_history(lbm_batch* b, int32_t id, double* time, double* force, siz
 int lbm_batch_bytes(const lbm_batch* b, size_t* used, size_t* budget);
 /* Launch shape of the bulk kernel for the lattices of one storage format (autotuner). */
 int lbm_batch_set_launch(lbm_batch* b, int32_t storage, int32_t block, int32_t cells_per_thread);
+/* Occupancy cap of the bulk launch of one storage group (lbm3d_set_occupancy, one cap for both
+ * step parities, since one launch advances lattices on either): 0 (no cap) to 32 workgroups per
+ * compute unit, refused if the device's LDS cannot express it. Results do not change. The
+ * autotuner leaves batches uncapped (a cap ran a batch of uneven lattices 2% slower). */
+int lbm_batch_set_occupancy(lbm_batch* b, int32_t storage, int32_t groups_per_cu);
 /* Autotunes the bulk launch shape of each storage format present in the batch on a scratch
  * copy of its largest lattice (lbm3d_autotune, serial schedule; the batch's lattices are not
- * advanced) and applies it with lbm_batch_set_launch. best[storage] (4 entries, indexed by
- * LBM_STORE_*) receives the choice, block 0 for a storage the batch does not hold. */
+ * advanced) and applies it with lbm_batch_set_launch, uncapped (lbm_batch_set_occupancy 0).
+ * best[storage] (4 entries, indexed by LBM_STORE_*) receives the choice as applied (groups 0),
+ * block 0 for a storage the batch does not hold. */
 int lbm_batch_autotune(lbm_batch* b, uint32_t steps, lbm_launch_choice* best);
 /* set_inlet_velocity of one lattice of the batch. */
 int lbm_batch_set_inlet(lbm_batch* b, int32_t id, const double velocity[3]);

@@ -186,6 +186,7 @@ EXPORTS = [
                                          c_sizep]),
     ("lbm_batch_bytes", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep]),
     ("lbm_batch_set_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    ("lbm_batch_set_occupancy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("lbm_batch_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, P(CLaunchChoice)]),
     ("lbm_batch_set_inlet", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p]),
     ("lbm_force_coefficients", None, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, c_f64p,

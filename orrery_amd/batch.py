@@ -116,6 +116,11 @@ class LatticeBatch:
     def set_launch(self, storage: str, block: int, cells_per_thread: int) -> None:
         _check(self._lib, self._lib.lbm_batch_set_launch(self._b, abi.STORAGE[storage], block, cells_per_thread))
 
+    def set_occupancy(self, storage: str, groups_per_cu: int) -> None:
+        """Caps the bulk launch of one storage group at `groups_per_cu` resident workgroups per
+        compute unit (0: no cap); results do not change."""
+        _check(self._lib, self._lib.lbm_batch_set_occupancy(self._b, abi.STORAGE[storage], int(groups_per_cu)))
+
     def bytes(self) -> tuple[int, int]:
         u = ctypes.c_size_t()
         t = ctypes.c_size_t()

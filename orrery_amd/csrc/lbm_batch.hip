@@ -237,6 +237,7 @@ struct Group {
     std::uint32_t link_blocks = 0;
     int block = 256;
     int cpt = 1;
+    std::uint32_t lds = 0; // dynamic LDS reserved per bulk block: the occupancy cap
 };
 
 } // namespace
@@ -265,6 +266,9 @@ struct lbm_batch {
     double* pinned = nullptr;
     std::size_t pinned_doubles = 0;
     std::map<int, std::pair<int, int>> launch; // storage -> (block, cpt)
+    std::map<int, int> occupancy;              // storage -> bulk groups per CU (0: no cap)
+    std::uint32_t lds_per_cu = 0;
+    std::uint32_t lds_per_block = 0;
 };
 
 namespace {
@@ -349,15 +353,15 @@ template <class T> void enqueue_step(Group& g, hipStream_t s) {
     const auto n = static_cast<std::uint32_t>(g.members.size());
     switch (g.cpt) {
         case 2:
-            hipLaunchKernelGGL((lbm::batch_bulk<T, 2>), dim3(g.bulk_blocks), dim3(g.block), 0, s, lat, g.bulk_start, n,
+            hipLaunchKernelGGL((lbm::batch_bulk<T, 2>), dim3(g.bulk_blocks), dim3(g.block), g.lds, s, lat, g.bulk_start, n,
                                g.parity);
             break;
         case 4:
-            hipLaunchKernelGGL((lbm::batch_bulk<T, 4>), dim3(g.bulk_blocks), dim3(g.block), 0, s, lat, g.bulk_start, n,
+            hipLaunchKernelGGL((lbm::batch_bulk<T, 4>), dim3(g.bulk_blocks), dim3(g.block), g.lds, s, lat, g.bulk_start, n,
                                g.parity);
             break;
         default:
-            hipLaunchKernelGGL((lbm::batch_bulk<T, 1>), dim3(g.bulk_blocks), dim3(g.block), 0, s, lat, g.bulk_start, n,
+            hipLaunchKernelGGL((lbm::batch_bulk<T, 1>), dim3(g.bulk_blocks), dim3(g.block), g.lds, s, lat, g.bulk_start, n,
                                g.parity);
             break;
     }
@@ -407,6 +411,10 @@ int rebuild(lbm_batch& b) {
         if (it != b.launch.end()) {
             g.block = it->second.first;
             g.cpt = it->second.second;
+        }
+        const auto cap = b.occupancy.find(st);
+        if (cap != b.occupancy.end() && cap->second > 0) {
+            g.lds = b.lds_per_cu / static_cast<std::uint32_t>(cap->second) / 512U * 512U;
         }
         b.groups.push_back(g);
     }
@@ -464,10 +472,24 @@ int lbm_batch_create(size_t budget_bytes, uint32_t sample_every, lbm_batch** out
     }
     b->budget = budget_bytes;
     b->sample_every = sample_every;
-    const hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    int device = 0;
+    int lds_cu = 0;
+    int lds_block = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e == hipSuccess) {
+        e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device);
+    }
+    if (e == hipSuccess) {
+        e = hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    }
+    b->lds_per_cu = static_cast<std::uint32_t>(lds_cu > 0 ? lds_cu : 0);
+    b->lds_per_block = static_cast<std::uint32_t>(lds_block > 0 ? lds_block : 0);
+    if (e == hipSuccess) {
+        e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    }
     if (e != hipSuccess) {
         delete b;
-        return hip_fail_b("stream", e);
+        return hip_fail_b("device query or stream", e);
     }
     *out = b;
     return ok();
@@ -686,6 +708,24 @@ int lbm_batch_set_launch(lbm_batch* b, int32_t storage, int32_t block, int32_t c
     return ok();
 }
 
+int lbm_batch_set_occupancy(lbm_batch* b, int32_t storage, int32_t groups_per_cu) {
+    if (b == nullptr || storage < 0 || storage > 3) {
+        return fail(LBM_EINVAL, "lbm batch: null handle or unknown storage");
+    }
+    if (groups_per_cu < 0 || groups_per_cu > 32) {
+        return fail(LBM_EINVAL, "lbm batch: workgroups per compute unit must be 0 (no cap) to 32");
+    }
+    if (groups_per_cu > 0 &&
+        (b->lds_per_cu == 0 || b->lds_per_cu / static_cast<std::uint32_t>(groups_per_cu) / 512U * 512U >
+                                   b->lds_per_block)) {
+        return fail(LBM_EINVAL, "lbm batch: the device's LDS cannot express a cap of " +
+                                    std::to_string(groups_per_cu) + " workgroups per compute unit");
+    }
+    b->occupancy[storage] = groups_per_cu;
+    b->dirty = true;
+    return ok();
+}
+
 int lbm_batch_autotune(lbm_batch* b, uint32_t steps, lbm_launch_choice* best) {
     if (b == nullptr || best == nullptr) {
         return fail(LBM_EINVAL, "lbm batch: null argument");
@@ -721,6 +761,13 @@ int lbm_batch_autotune(lbm_batch* b, uint32_t steps, lbm_launch_choice* best) {
         }
         if (rc == LBM_OK) {
             rc = lbm_batch_set_launch(b, st, best[st].block, best[st].cells_per_thread);
+        }
+        if (rc == LBM_OK) {
+            // The caps tuned on one lattice are not applied: on a batch of uneven lattices they
+            // measured slower (48 wing sections: 4 groups per CU 101.8 s, uncapped 99.6 s).
+            best[st].groups_a = 0;
+            best[st].groups_b = 0;
+            rc = lbm_batch_set_occupancy(b, st, 0);
         }
         if (rc != LBM_OK) {
             return rc;

@@ -76,6 +76,7 @@ def test_tuned_batch_gives_the_untuned_histories():
             print(f"batch choices: {chosen}", flush=True)
             assert set(chosen) == {"f32", "f16s"}
             assert all(ch.schedule == "serial" for ch in chosen.values())
+            assert all(ch.groups_a == ch.groups_b == 0 for ch in chosen.values())
         b.run(60)
         runs.append([lat.history() for lat in lats])
         b.close()

@@ -47,6 +47,26 @@ def test_batch_matches_single_lattices_bitwise():
             assert tuple(f[k, :3]) == fs.force and tuple(f[k, 3:]) == fs.moment
 
 
+@pytest.mark.parametrize("groups", [3, 5])
+def test_occupancy_caps_leave_the_batch_bitwise_unchanged(groups):
+    steps = 40
+    batch = LatticeBatch(sample_every=10)
+    members = [(batch.add(c, m, st, seed, 1e-4), c, m, st, seed) for c, m, st, seed in lattices()]
+    for st in ("f32", "f16s", "bf16s"):
+        batch.set_occupancy(st, groups)
+    batch.run(steps)
+    for lat, c, m, st, seed in members:
+        single, _ = alone(c, m, st, seed, steps)
+        assert np.array_equal(lat.populations(), single.populations())
+
+
+def test_occupancy_caps_are_checked():
+    batch = LatticeBatch(sample_every=10)
+    for bad in (-1, 33):
+        with pytest.raises(LbmError, match="workgroups per compute unit"):
+            batch.set_occupancy("f32", bad)
+
+
 def test_a_lattice_is_the_same_alone_and_in_the_mix():
     c, m, st, seed = lattices()[1]
     solo = LatticeBatch(sample_every=10)
