@@ -121,8 +121,9 @@ def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare
     return out
 
 
-def cpu_baseline(config, mask, steps):
-    """The oracle on the same lattice: one warm-up step, then `steps` timed steps."""
+def cpu_baseline(config, mask, steps, seconds=0.0):
+    """The oracle on the same lattice: one warm-up step, then `steps` timed steps, repeated until
+    at least `seconds` have passed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import Oracle3D, load
     threads = os.cpu_count() or 1
@@ -131,8 +132,14 @@ def cpu_baseline(config, mask, steps):
     oracle.initialize_perturbed(2024, 1e-4)
     oracle.step(1)
     t0 = time.perf_counter()
-    oracle.step(steps)
-    dt = time.perf_counter() - t0
+    done = 0
+    while True:
+        oracle.step(steps)
+        done += steps
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    steps = done
     del oracle
     cells = config.cells
     return {"value": cells * steps / dt / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
@@ -560,7 +567,10 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=10,
+                    help="oracle steps per chunk of the CPU baseline (0: no CPU baseline)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="the CPU baseline repeats its chunk until this many seconds have passed")
     ap.add_argument("--storages", default="f32,f16s,bf16s")
     ap.add_argument("--tune-steps", type=int, default=30,
                     help="steps per candidate of the launch autotuner before timing (0: the default launch)")
@@ -638,7 +648,7 @@ def main(argv=None):
     cpu = None
     if args.cpu_steps > 0:
         print("bench: timing the CPU baseline (oracle)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(config, mask, args.cpu_steps)
+        cpu = cpu_baseline(config, mask, args.cpu_steps, args.cpu_seconds)
     traffic = measured_traffic()
     line = {
         "metric": "D3Q19 GLUPS (FP32 A-A, BGK+Smagorinsky) and % HBM roofline",
