@@ -72,8 +72,8 @@ def sphere_variants():
 @pytest.mark.parametrize("variant", ["bgk", "les", "interpolated", "extrapolation_periodic_y"])
 def test_laminar_sphere_channel_f32(variant):
     # test_lbm3d_gpu.cpp:211-238: Re 20, D 8, 64x32x32, 800 steps, every metric < 1e-4
-    configs, mask = sphere_variants()
-    out, _, _ = compare(configs[variant], mask, 800, "f32", variant)
+    variants, mask = sphere_variants()
+    out, _, _ = compare(variants[variant], mask, 800, "f32", variant)
     assert out["velocity"] < 1e-4
     assert out["populations"] < 1e-4
     assert out["force_x"] < 1e-4
@@ -81,8 +81,8 @@ def test_laminar_sphere_channel_f32(variant):
 
 @pytest.mark.parametrize("variant", ["bgk", "les", "interpolated", "extrapolation_periodic_y"])
 def test_laminar_sphere_channel_f64_matches_the_oracle_to_rounding(variant):
-    configs, mask = sphere_variants()
-    out, cpu, gpu = compare(configs[variant], mask, 201, "f64", variant)
+    variants, mask = sphere_variants()
+    out, cpu, gpu = compare(variants[variant], mask, 201, "f64", variant)
     assert out["velocity"] < 1e-12
     assert out["populations"] < 1e-13
     assert out["force_x"] < 1e-11
@@ -282,8 +282,8 @@ def test_refusals_happen_before_allocation():
 
 @pytest.mark.parametrize("storage,limit", [("f16s", 5e-3), ("bf16s", 5e-2)])
 def test_sixteen_bit_storage_error_against_fp64(storage, limit):
-    configs, mask = sphere_variants()
-    c = configs["bgk"]
+    variants, mask = sphere_variants()
+    c = variants["bgk"]
     ref = Lbm3DSolver(c, mask, "f64")
     low = Lbm3DSolver(c, mask, storage)
     ref.step(800)

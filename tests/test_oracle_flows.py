@@ -26,7 +26,8 @@ def run_poiseuille(tau, steps):
 
 
 def test_poiseuille_golden_value_recorded_from_the_reference():
-    # Reference build: 4x24x4, tau = 1/2 + sqrt(3/16), g = 1e-6, 2000 steps:
+    # Recorded from a build of the reference's own solver: 4x24x4, tau = 1/2 + sqrt(3/16), g = 1e-6,
+    # 2000 steps:
     # u(y = 12) = 4.9431445994122588e-4. Bitwise: the restatement keeps the reference's
     # expression order and -ffp-contract=off.
     _, m = run_poiseuille(MAGIC_TAU, 2000)

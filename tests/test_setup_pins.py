@@ -56,7 +56,7 @@ def c2():
 
 
 def test_c1_c2_counts_match_the_reference_record():
-    # SURVEY Appendix A (reference build): C1 797 solid / 312 links / tau 0.548;
+    # Recorded from a build of the reference's own solver: C1 797 solid / 312 links / tau 0.548;
     # C2 17,077 solid / 18,282 links / tau 0.524.
     c, m = c1()
     assert m.count() == 797
