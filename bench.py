@@ -600,7 +600,7 @@ def main(argv=None):
                          "known-size reference for the HBM byte counters of a profiler run")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
               f"(independent lattices shard across GPUs by HBM budget, one lattice stays on one GPU)",
               file=sys.stderr)
         return 2

@@ -2,7 +2,7 @@
 // solver's algorithm for the D3Q19 and D2Q9 lattice Boltzmann step (pull streaming into a
 // second buffer, every boundary rule resolved per link, BGK + Guo forcing + Smagorinsky +
 // absorbing layers). Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load
-// it, and only as the checker; the product library never links it.
+// it, and only to check results; the product library never links it.
 //
 // Each function names the reference lines it restates (paths relative to the reference
 // root). Expression order follows the reference so that rounding matches; the library is
