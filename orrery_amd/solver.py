@@ -17,6 +17,15 @@ from .config import Lbm2DConfig, Lbm3DConfig
 from .masks import SolidMask2D, SolidMask3D
 
 
+@dataclasses.dataclass(frozen=True)
+class NodeCounts:
+    """Node classes of a D3Q19 lattice (lbm3d_counts)."""
+    bulk_nodes: int
+    boundary_nodes: int
+    solid_nodes: int
+    links: int
+
+
 class LbmError(RuntimeError):
     """A refused configuration or a failed device operation (code and message of the ABI)."""
 
@@ -228,6 +237,12 @@ class Lbm3DSolver:
         a, b = ctypes.c_int32(), ctypes.c_int32()
         _check(self._lib, self._lib.lbm3d_get_occupancy(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def counts(self) -> "NodeCounts":
+        """lbm3d_counts: bulk nodes, boundary nodes, solid nodes and solid links of the lattice."""
+        v = [ctypes.c_size_t() for _ in range(4)]
+        _check(self._lib, self._lib.lbm3d_counts(self._h, *[ctypes.byref(x) for x in v]))
+        return NodeCounts(*(x.value for x in v))
 
     def set_schedule(self, schedule: str) -> None:
         """"serial" (bulk then boundary kernel, the default), "fused" (one kernel per step) or

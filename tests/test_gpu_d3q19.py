@@ -256,6 +256,42 @@ def test_momentum_balance_on_device():
     assert f.force[0] == pytest.approx(2e-6 * 240, rel=2e-3)
 
 
+def shear_wave(c, amplitude):
+    """rho 1, u_x = amplitude sin(2 pi y / ny), u_y = u_z = 0 on every cell."""
+    planes = np.zeros((4, c.cells))
+    y = (np.arange(c.cells) // c.nx) % c.ny
+    planes[0] = 1.0
+    planes[1] = amplitude * np.sin(2.0 * math.pi * y / c.ny)
+    return planes
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_fully_periodic_box_decays_like_the_shear_wave(storage):
+    # Every side periodic, no solid: the nodes on the seams (whose pulls wrap) are boundary nodes,
+    # the interior bulk nodes. The shear wave decays as exp(-nu k^2 t), nu = (tau - 1/2) / 3,
+    # k = 2 pi / ny, up to the lattice's O(k^2) dispersion; binary64 agrees with the oracle to
+    # rounding.
+    c = periodic_box(8, 32, 6, 0.8)
+    amplitude, steps = 1e-3, 400
+    planes = shear_wave(c, amplitude)
+    out, cpu, gpu = compare(c, fluid_mask_3d(8, 32, 6), steps, storage, "shear wave", planes)
+    counts = gpu.counts()
+    interior = (c.nx - 2) * (c.ny - 2) * (c.nz - 2)
+    assert counts.bulk_nodes == interior and counts.boundary_nodes == c.cells - interior
+    assert counts.solid_nodes == 0 and counts.links == 0
+    k = 2.0 * math.pi / c.ny
+    want = amplitude * math.exp(-(c.tau - 0.5) / 3.0 * k * k * steps)
+    m = gpu.macroscopic()
+    y = (np.arange(c.cells) // c.nx) % c.ny
+    got = np.asarray(m.velocity_x)[y == 8].mean()  # sin = 1 at y = ny / 4
+    print(f"measured: shear wave {storage}: amplitude {got:.6e}, exp(-nu k^2 t) {want:.6e}", flush=True)
+    assert got == pytest.approx(want, rel=1e-2)
+    if storage == "f64":
+        assert out["velocity"] < 1e-12 and out["populations"] < 1e-12
+    else:
+        assert out["velocity"] < 1e-4  # the reference's GPU-vs-CPU bar (relative L2)
+
+
 def test_mass_conservation_f64():
     from test_oracle_flows import mass_case
     c, mask, planes = mass_case()

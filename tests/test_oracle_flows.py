@@ -331,3 +331,19 @@ def test_configurations_are_validated():
     assert validate3(ch, mask) == "lbm3d: the convective outlet needs a positive inlet speed"
     ch.outlet = Outlet.extrapolation
     assert validate3(ch, mask) is None
+
+
+def test_shear_wave_decays_at_the_lattice_viscosity():
+    # Closed form: a shear wave u_x = A sin(k y) in a fully periodic box decays as
+    # exp(-nu k^2 t) with nu = (tau - 1/2) / 3, up to the lattice's O(k^2) dispersion (0.5% here).
+    c = periodic_box(8, 32, 6, 0.8)
+    y = (np.arange(c.cells) // c.nx) % c.ny
+    planes = np.zeros((4, c.cells))
+    planes[0] = 1.0
+    planes[1] = 1e-3 * np.sin(2.0 * math.pi * y / c.ny)
+    o = Oracle3D(c, fluid_mask_3d(8, 32, 6))
+    o.initialize(planes)
+    o.step(400)
+    got = np.asarray(o.macroscopic().velocity_x)[y == 8].mean()
+    k = 2.0 * math.pi / c.ny
+    assert got == pytest.approx(1e-3 * math.exp(-(c.tau - 0.5) / 3.0 * k * k * 400), rel=1e-2)
