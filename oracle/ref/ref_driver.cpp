@@ -21,6 +21,7 @@
 #include "orrery/aero/lbm/solid_mask.h"
 #include "orrery/aero/lbm3d/lbm3d.h"
 #include "orrery/aero/lbm3d/solid_mask3d.h"
+#include "orrery/geom/reference_quantities.h"
 #include "orrery/geom/naca.h"
 
 using namespace orrery::aero::lbm;
@@ -412,6 +413,21 @@ void ref_voxelize(const double* v, std::size_t nv, const std::uint32_t* t, std::
     const VoxelGrid vg = voxelize(m, g, threshold);
     std::memcpy(solid, vg.solid.data(), vg.solid.size());
     std::memcpy(winding, vg.winding.data(), vg.winding.size() * sizeof(double));
+}
+
+// reference_quantities (src/geom/reference_quantities.cpp) with flow +x and lift +z: out = wetted,
+// frontal, planform, frontal bound, planform bound, reference length, span, volume, centre x/y/z.
+void ref_reference_quantities(const double* v, std::size_t nv, const std::uint32_t* t, std::size_t nt,
+                              std::size_t resolution, double* out) {
+    using namespace orrery::geom;
+    const TriangleMesh m = mesh_of(v, nv, t, nt);
+    ReferenceOptions o;
+    o.raster_resolution = resolution;
+    const ReferenceQuantities q = reference_quantities(m, o);
+    const double values[11] = {q.wetted_area, q.frontal_area, q.planform_area, q.frontal_area_bound,
+                               q.planform_area_bound, q.reference_length, q.span, q.volume,
+                               q.centre_of_mass.x, q.centre_of_mass.y, q.centre_of_mass.z};
+    std::memcpy(out, values, sizeof(values));
 }
 
 void ref_signed_distance_band(const double* v, std::size_t nv, const std::uint32_t* t, std::size_t nt,

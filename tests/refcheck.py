@@ -65,6 +65,7 @@ def load():
                                                                          P(ctypes.c_size_t)]),
         "ref_voxelize": (None, [_f64p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _f64p, ctypes.c_double, ctypes.c_size_t,
                                 ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double, _u8p, _f64p]),
+        "ref_reference_quantities": (None, [_f64p, ctypes.c_size_t, _u32p, ctypes.c_size_t, ctypes.c_size_t, _f64p]),
         "ref_signed_distance_band": (None, [_f64p, ctypes.c_size_t, _u32p, ctypes.c_size_t, _f64p, ctypes.c_double,
                                             ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _u8p, ctypes.c_double,
                                             _f64p]),
@@ -219,3 +220,17 @@ def signed_distance_band(v, t, origin, spacing, nx, ny, nz, solid, band_cells=3.
                                     ptr(np.ascontiguousarray(solid, dtype=np.uint8), ctypes.c_uint8), band_cells,
                                     ptr(d))
     return d
+
+
+REFERENCE_QUANTITY_NAMES = ("wetted_area", "frontal_area", "planform_area", "frontal_area_bound",
+                            "planform_area_bound", "reference_length", "span", "volume", "centre_x", "centre_y",
+                            "centre_z")
+
+
+def reference_quantities(v, t, resolution):
+    """reference_quantities of the reference (flow +x, lift +z) on a mesh, as a dict."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.uint32)
+    out = np.empty(len(REFERENCE_QUANTITY_NAMES))
+    load().ref_reference_quantities(ptr(v), len(v), ptr(t, ctypes.c_uint32), len(t), resolution, ptr(out))
+    return dict(zip(REFERENCE_QUANTITY_NAMES, out.tolist()))

@@ -47,6 +47,36 @@ def test_silhouettes_against_the_convex_bound(shape, dims):
         assert area == pytest.approx(silhouette_bound(p.mesh, d), rel=1e-2)
 
 
+def _reference_quantities_cases():
+    from golden.make_reference_quantities import CASES, RESOLUTIONS
+    return [(key, shape, dims, res) for key, (shape, dims) in CASES.items() for res in RESOLUTIONS]
+
+
+@pytest.mark.parametrize("key,shape,dims,res", _reference_quantities_cases())
+def test_quantities_match_the_reference(key, shape, dims, res):
+    """runner.reference_quantities against the reference's reference_quantities
+    (src/geom/reference_quantities.cpp, built into oracle/_ref; the committed fixture
+    tests/golden/reference_quantities.json where it is not built): the rasterised silhouettes bit
+    for bit, the sums to rounding."""
+    import json
+    import os
+    import refcheck
+    mesh = runner.prepare_geometry(ref(shape, **dims)).mesh
+    if refcheck.available():
+        want = refcheck.reference_quantities(mesh.vertices, mesh.triangles, res)
+    else:
+        with open(os.path.join(os.path.dirname(__file__), "golden", "reference_quantities.json")) as f:
+            want = json.load(f)[f"{key}@{res}"]
+    q = runner.reference_quantities(mesh, res)
+    assert q.frontal_area == want["frontal_area"] and q.planform_area == want["planform_area"]
+    assert q.reference_length == want["reference_length"]
+    assert q.wetted_area == pytest.approx(want["wetted_area"], rel=1e-13)
+    assert q.volume == pytest.approx(want["volume"], rel=1e-13)
+    size = want["reference_length"]
+    for got, name in zip(q.centre_of_mass, ("centre_x", "centre_y", "centre_z")):
+        assert got == pytest.approx(want[name], abs=1e-13 * size)
+
+
 def test_sphere_quantities_against_the_closed_form():
     r = 0.65
     q = runner.prepare_geometry(ref(ex.AeroShape.sphere, diameter_m=2 * r)).quantities
