@@ -71,7 +71,7 @@ def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare
         launch = {"schedule": best.schedule, "block": best.block, "cells_per_thread": best.cells_per_thread,
                   "groups_per_cu": [best.groups_a, best.groups_b],
                   "tuned_ms_per_step": round(best.ms_per_step, 4),
-                  "default_ms_per_step": round(next(t.ms_per_step for t in tried if t.schedule == "serial" and t.block == 256
+                  "default_ms_per_step": round(next(t.ms_per_step for t in reversed(tried) if t.schedule == "serial" and t.block == 256
                                                     and t.cells_per_thread == (2 if storage in ("f16s", "bf16s") else 1)
                                                     and t.groups_a == 0 and t.groups_b == 0), 4),
                   "candidates": len(tried)}

@@ -185,8 +185,8 @@ typedef struct lbm_launch_choice {
  * bulk launch shape (block 64/128/192/256 x 1/2/4 cells per thread) of the serial schedule, and
  * the fused schedule unless serial_only; under the fastest serial shape it then tries caps of
  * 5, 4 and 3 workgroups per compute unit on the A step, then on the B step, then on the A step
- * again (lbm3d_set_occupancy); the four fastest are timed once more, and the fastest of that
- * final round is kept on the handle. The lattice is
+ * again (lbm3d_set_occupancy); the four fastest, and the default launch if it is not among
+ * them, are timed once more, and the fastest of that final round is kept on the handle. The lattice is
  * copied on the device first and restored afterwards, with its step count: its later results
  * are those it would have had without tuning, bit for bit (all schedules and shapes compute
  * the same values). Needs device memory for that copy (LBM_ENOMEM otherwise). `tried` (capacity

@@ -921,6 +921,17 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
                 finals.push_back(c);
             }
         }
+        // The default launch competes in the final round too, so that the choice is never one
+        // that the final round timed slower than the default.
+        const int default_cpt = lbm::internal::default_cells_per_thread(h->storage);
+        bool has_default = false;
+        for (const auto& f : finals) {
+            has_default = has_default || (f.schedule == LBM_SCHEDULE_SERIAL && f.block == 256 &&
+                                          f.cells_per_thread == default_cpt && f.groups_a == 0 && f.groups_b == 0);
+        }
+        if (!has_default) {
+            finals.push_back({LBM_SCHEDULE_SERIAL, 256, default_cpt, 0, 0, 0.0});
+        }
         for (auto& c : finals) {
             if (!time_candidate(c)) {
                 break;

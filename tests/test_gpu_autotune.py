@@ -34,10 +34,13 @@ def test_tuning_mid_run_changes_no_result(storage):
     # 13 shapes, then 3 A-step, 3 B-step and 3 A-step occupancy caps when a serial shape is
     # fastest, then the fastest four timed again: the choice is the fastest of that final round
     first = 22 if min(tried[:13], key=lambda t: t.ms_per_step).schedule == "serial" else 13
-    assert len(tried) == first + 4
+    # (and the default launch, if it is not among them)
+    assert len(tried) in (first + 4, first + 5)
     final = tried[first:]
     assert best.ms_per_step == min(t.ms_per_step for t in final)
-    assert len({(t.schedule, t.block, t.cells_per_thread, t.groups_a, t.groups_b) for t in final}) == 4
+    shapes = {(t.schedule, t.block, t.cells_per_thread, t.groups_a, t.groups_b) for t in final}
+    assert len(shapes) == len(final)
+    assert any(s[0] == "serial" and s[1] == 256 and s[3] == s[4] == 0 for s in shapes)
     assert autotune.current(tuned) == (best.schedule, best.block, best.cells_per_thread)
     assert tuned.occupancy() == (best.groups_a, best.groups_b)
     assert all(t.groups_a in (0, 3, 4, 5) and t.groups_b in (0, 3, 4, 5) for t in tried)
