@@ -184,7 +184,7 @@ typedef struct lbm_launch_choice {
 /* Autotuner (no reference counterpart): times `steps` steps (rounded up to even) under every
  * bulk launch shape (block 64/128/192/256 x 1/2/4 cells per thread) of the serial schedule, and
  * the fused schedule unless serial_only; under the fastest serial shape it then tries caps of
- * 5, 4 and 3 workgroups per compute unit on the A step, then on the B step, then on the A step
+ * 7, 6, 5, 4 and 3 workgroups per compute unit on the A step, then on the B step, then on the A step
  * again (lbm3d_set_occupancy); the four fastest, and the default launch if it is not among
  * them, are timed once more, and the fastest of that final round is kept on the handle. The lattice is
  * copied on the device first and restored afterwards, with its step count: its later results

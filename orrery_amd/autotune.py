@@ -91,7 +91,7 @@ def tune(solver: Lbm3DSolver, steps: int = 10, serial_only: bool = False) -> tup
     """Times every candidate on the solver's lattice (restored afterwards) and applies the
     fastest; returns it and every candidate timed."""
     lib = abi.load()
-    tried = (abi.CLaunchChoice * 32)()
+    tried = (abi.CLaunchChoice * 64)()
     n = ctypes.c_size_t()
     best = abi.CLaunchChoice()
     _check(lib, lib.lbm3d_autotune(solver.handle, int(steps), 1 if serial_only else 0, tried, len(tried),

@@ -886,7 +886,7 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
         for (int pass = 0; pass < 3 && rc == LBM_OK; ++pass) {
             const int side = pass & 1;
             const lbm_launch_choice base = cand[fastest()];
-            for (const int g : {5, 4, 3}) {
+            for (const int g : {7, 6, 5, 4, 3}) {
                 lbm_launch_choice c = base;
                 (side == 0 ? c.groups_a : c.groups_b) = g;
                 if (h->lds_per_cu / static_cast<std::uint32_t>(g) / 512U * 512U > h->lds_per_block) {
