@@ -254,7 +254,9 @@ int lbm_batch_remove(lbm_batch* b, int32_t id);
 int lbm_batch_run(lbm_batch* b, uint64_t steps);
 int lbm_batch_count(const lbm_batch* b, size_t* count);
 /* The lattice's handle (owned by the batch): lbm3d_force/_macroscopic/_populations/_steps
- * work on it between runs. */
+ * work on it between runs. The calls that would step, re-initialise, retarget or free it
+ * (lbm3d_step, _initialize*, _set_inlet, _time_steps, _autotune, _destroy) refuse it with
+ * LBM_EINVAL: the batch keeps its A-A parity on the device and steps it alone. */
 int lbm_batch_lattice(lbm_batch* b, int32_t id, lbm3d** h);
 /* Force samples of a lattice: time (step count) and 6 doubles each (force xyz, moment xyz). */
 int lbm_batch_history(lbm_batch* b, int32_t id, double* time, double* force, size_t capacity,
@@ -275,6 +277,20 @@ int lbm_batch_set_occupancy(lbm_batch* b, int32_t storage, int32_t groups_per_cu
 int lbm_batch_autotune(lbm_batch* b, uint32_t steps, lbm_launch_choice* best);
 /* set_inlet_velocity of one lattice of the batch. */
 int lbm_batch_set_inlet(lbm_batch* b, int32_t id, const double velocity[3]);
+/* Per-kernel times of a batch run (HIP events on the batch's stream), indexed by LBM_STORE_*
+ * where per storage group. */
+typedef struct lbm_batch_timing {
+    double step_ms;          /* wall time of the run / steps (force samples included) */
+    double bulk_ms[4];       /* bulk kernel of the group, mean per step (0: no such group) */
+    double boundary_ms[4];   /* boundary kernel of the group, mean per step */
+    double force_ms;         /* force kernels of every group, mean per sample */
+    uint64_t bulk_nodes[4];  /* bulk nodes the group's bulk launch updates */
+    uint64_t boundary_nodes[4];
+} lbm_batch_timing;
+/* lbm_batch_run with every kernel launched on its own (no graph replay) between HIP events:
+ * the same kernels in the same order, so the lattices and their histories advance exactly as
+ * lbm_batch_run would advance them. */
+int lbm_batch_run_timed(lbm_batch* b, uint64_t steps, lbm_batch_timing* timing);
 
 /* ---- host-only setup (no device needed) --------------------------------------------------- */
 /* validate() (src/aero/lbm3d/lbm3d_common.cpp:159-161): LBM_OK, or LBM_EINVAL with the
@@ -314,6 +330,12 @@ int lbm_steady_since(const double* time, const double* drag, const double* lift,
 int lbm_settled_oscillation(const double* time, const double* drag, const double* lift, size_t n,
                             double from, double max_period_spread, double stationarity,
                             lbm_oscillation* out);
+/* forces_settled_3d (src/aero/lbm3d/lbm_aero_solver3d.cpp:60-83): 1 when drag, lift and side
+ * force are each steady over the samples at time >= from (at least two samples, width within
+ * steady_tolerance x |mean drag|) or settled periodic shedding (lbm_settled_oscillation of the
+ * drag with that component), else 0. */
+int lbm_forces_settled(const double* time, const double* drag, const double* lift, const double* side, size_t n,
+                       double from, double steady_tolerance, double max_period_spread, double stationarity);
 
 /* ---- geometry: meshes, voxelisation, signed distance (src/geom) --------------------------- */
 /* Host-only, deterministic. Winding numbers follow MeshTree (winding_number.cpp: Barill et al.

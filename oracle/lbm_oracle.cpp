@@ -610,6 +610,10 @@ public:
     const lbm3d_config& config() const { return c_; }
     const std::vector<double>& populations() const { return f_; }
     const std::vector<Link3>& links() const { return links_; }
+    // Test hooks: any population state (the logical layout f[i * cells + cell]), and the
+    // convective-outlet history [5][nz][ny] (slot-major).
+    void set_populations(const double* f) { std::copy(f, f + f_.size(), f_.begin()); }
+    const std::vector<double>& outlet_history() const { return outlet_; }
     std::uint64_t steps() const { return steps_; }
     void set_inlet(const double* v) { inlet_ = {v[0], v[1], v[2]}; }
     const std::array<double, 3>& inlet() const { return inlet_; }
@@ -1269,6 +1273,14 @@ void orc3d_macroscopic(void* h, double* rho, double* ux, double* uy, double* uz)
 void orc3d_populations(void* h, double* f) {
     const std::vector<double>& v = static_cast<Oracle3D*>(h)->populations();
     std::memcpy(f, v.data(), v.size() * sizeof(double));
+}
+void orc3d_set_populations(void* h, const double* f) { static_cast<Oracle3D*>(h)->set_populations(f); }
+std::size_t orc3d_outlet_history(void* h, double* out) {
+    const std::vector<double>& v = static_cast<Oracle3D*>(h)->outlet_history();
+    if (out != nullptr) {
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    }
+    return v.size();
 }
 std::size_t orc3d_links(void* h, std::uint32_t* cell, std::uint32_t* dir, double* q, std::size_t cap) {
     const std::vector<Link3>& links = static_cast<Oracle3D*>(h)->links();

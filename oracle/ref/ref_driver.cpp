@@ -131,6 +131,37 @@ std::size_t ref3d_boundary_links(const lbm3d_config* c, const std::uint8_t* soli
     return links.size();
 }
 
+// The reference's Bouzidi reflection (lbm3d_internal.h:185-197) on a caller's state.
+double ref3d_reflected(const double* f, std::size_t cells, std::size_t here, std::size_t farther, std::size_t i,
+                       double q) {
+    const std::vector<double> v(f, f + 19 * cells);
+    return detail3d::reflected(v, cells, here, farther, i, q);
+}
+
+// Lbm3DSolver::force of the CPU solver (lbm3d_cpu.cpp:192-203: for each boundary link, the
+// reflected population and add_link_force / link_wall_point, lbm3d_common.cpp:213-241) over a
+// caller's population state f[i * cells + cell] and the reference's own boundary links.
+void ref3d_force_of(const lbm3d_config* c, const std::uint8_t* solid, const double* distance, const double* f,
+                    double* force, double* moment) {
+    const Lbm3DConfig config = config3(*c);
+    const SolidMask3D mask = mask3(*c, solid, distance);
+    const std::size_t cells = c->nx * c->ny * c->nz;
+    const std::vector<double> v(f, f + 19 * cells);
+    const auto offset = detail3d::source_offsets(config);
+    Force3D total;
+    for (const BoundaryLink3D& link : boundary_links(config, mask)) {
+        const std::size_t out = d3q19::opposite[link.direction];
+        const auto farther =
+            static_cast<std::size_t>(static_cast<std::ptrdiff_t>(link.cell) - offset[link.direction]);
+        const double back = detail3d::reflected(v, cells, link.cell, farther, link.direction, link.q);
+        add_link_force(total, config, link, v[out * cells + link.cell] + back);
+    }
+    for (int a = 0; a < 3; ++a) {
+        force[a] = total.force[a];
+        moment[a] = total.moment[a];
+    }
+}
+
 void ref3d_classify(const lbm3d_config* c, const std::uint8_t* solid, std::uint8_t* out) {
     const auto classes = detail3d::classify_nodes(config3(*c), mask3(*c, solid, nullptr));
     std::memcpy(out, classes.data(), classes.size());

@@ -66,6 +66,12 @@ U64 = ctypes.c_uint64
 I32 = ctypes.c_int32
 
 
+class CBatchTiming(ctypes.Structure):
+    """lbm_batch_timing"""
+    _fields_ = [("step_ms", D), ("bulk_ms", D * 4), ("boundary_ms", D * 4), ("force_ms", D),
+                ("bulk_nodes", U64 * 4), ("boundary_nodes", U64 * 4)]
+
+
 class CAeroOptions(ctypes.Structure):
     """lbm_aero3d_options (LbmSolver3DOptions, lbm_aero_solver3d.h:45-81)."""
     _fields_ = [("cells_per_length", D), ("lattice_velocity", D), ("upstream", D), ("downstream", D), ("width", D),
@@ -180,6 +186,7 @@ EXPORTS = [
                                      ctypes.c_double, P(ctypes.c_int32)]),
     ("lbm_batch_remove", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("lbm_batch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    ("lbm_batch_run_timed", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(CBatchTiming)]),
     ("lbm_batch_count", ctypes.c_int, [ctypes.c_void_p, c_sizep]),
     ("lbm_batch_lattice", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_void_p)]),
     ("lbm_batch_history", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_f64p, c_f64p, ctypes.c_size_t,
@@ -198,6 +205,8 @@ EXPORTS = [
                                         ctypes.c_double]),
     ("lbm_settled_oscillation", ctypes.c_int, [c_f64p, c_f64p, c_f64p, ctypes.c_size_t, ctypes.c_double,
                                                ctypes.c_double, ctypes.c_double, P(COscillation)]),
+    ("lbm_forces_settled", ctypes.c_int, [c_f64p, c_f64p, c_f64p, c_f64p, ctypes.c_size_t, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     ("lbm_geom_last_error", ctypes.c_char_p, []),
     ("lbm_mesh_create", ctypes.c_int, [c_f64p, ctypes.c_size_t, P(ctypes.c_uint32), ctypes.c_size_t,
                                        P(ctypes.c_void_p)]),

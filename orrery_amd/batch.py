@@ -113,6 +113,20 @@ class LatticeBatch:
     def run(self, steps: int) -> None:
         _check(self._lib, self._lib.lbm_batch_run(self._b, int(steps)))
 
+    def run_timed(self, steps: int) -> dict:
+        """run(steps) with each kernel between HIP events on the batch's stream (no graph
+        replay; the lattices advance exactly as run() advances them). Returns the mean step
+        time and, per storage format present, the bulk and boundary kernel times per step and
+        their node counts, plus the force sample time."""
+        t = abi.CBatchTiming()
+        _check(self._lib, self._lib.lbm_batch_run_timed(self._b, int(steps), ctypes.byref(t)))
+        groups = {}
+        for name, code in abi.STORAGE.items():
+            if t.bulk_nodes[code] > 0 or t.bulk_ms[code] > 0:
+                groups[name] = {"bulk_ms": t.bulk_ms[code], "boundary_ms": t.boundary_ms[code],
+                                "bulk_nodes": int(t.bulk_nodes[code]), "boundary_nodes": int(t.boundary_nodes[code])}
+        return {"step_ms": t.step_ms, "force_ms": t.force_ms, "groups": groups}
+
     def set_launch(self, storage: str, block: int, cells_per_thread: int) -> None:
         _check(self._lib, self._lib.lbm_batch_set_launch(self._b, abi.STORAGE[storage], block, cells_per_thread))
 

@@ -105,29 +105,10 @@ void apply_force(const lbm_aero3d& a, const Force& lattice, lbm_aero3d_result& r
     set_coefficients(to_body(a, f), to_body(a, m), a.conditions, a.reference, r);
 }
 
-// forces_settled_3d (lbm_aero_solver3d.cpp:60-83).
+// forces_settled_3d (lbm_aero_solver3d.cpp:60-83), with the reference's SettledCriteria.
 bool forces_settled(const lbm_aero3d& a, double from) {
-    const std::size_t n = a.time.size();
-    const double bound = kSteadyResidual * std::abs(lbm_mean_since(a.time.data(), a.drag.data(), n, from));
-    const auto stationary = [&](const std::vector<double>& values) {
-        std::size_t count = 0;
-        double lo = 0.0;
-        double hi = 0.0;
-        for (std::size_t k = 0; k < n; ++k) {
-            if (a.time[k] >= from) {
-                lo = count == 0 ? values[k] : std::min(lo, values[k]);
-                hi = count == 0 ? values[k] : std::max(hi, values[k]);
-                ++count;
-            }
-        }
-        if (count >= 2 && hi - lo <= bound) {
-            return true;
-        }
-        lbm_oscillation o{};
-        return lbm_settled_oscillation(a.time.data(), a.drag.data(), values.data(), n, from, kMaxPeriodSpread,
-                                       kStationarity, &o) == 1;
-    };
-    return stationary(a.drag) && stationary(a.lift) && stationary(a.side);
+    return lbm_forces_settled(a.time.data(), a.drag.data(), a.lift.data(), a.side.data(), a.time.size(), from,
+                              kSteadyResidual, kMaxPeriodSpread, kStationarity) == 1;
 }
 
 // The setup of lbm_aero_solver3d.cpp:85-339 up to (not including) the creation of the lattice.

@@ -169,6 +169,11 @@ int initialize_mode(lbm3d* h, int mode, const double* host_planes, std::uint64_t
 
 } // namespace
 
+// A lattice held by a batch is stepped by the batch alone (lbm_internal.h, lbm3d::in_batch).
+static int refuse_in_batch(const char* what) {
+    return fail(LBM_EINVAL, std::string("lbm3d: ") + what + ": the lattice belongs to a batch (use the lbm_batch_* calls)");
+}
+
 extern "C" {
 
 int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double* distance, int32_t storage,
@@ -342,15 +347,24 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
 }
 
 int lbm3d_destroy(lbm3d* h) {
+    if (h != nullptr && h->in_batch) {
+        return refuse_in_batch("destroy");
+    }
     release(h);
     return ok();
 }
 
 int lbm3d_initialize(lbm3d* h, const double* moments) {
+    if (h != nullptr && h->in_batch) {
+        return refuse_in_batch("initialize");
+    }
     return initialize_mode(h, moments != nullptr ? 1 : 0, moments, 0, 0.0);
 }
 
 int lbm3d_initialize_perturbed(lbm3d* h, uint64_t seed, double amplitude) {
+    if (h != nullptr && h->in_batch) {
+        return refuse_in_batch("initialize");
+    }
     if (!std::isfinite(amplitude)) {
         return fail(LBM_EINVAL, "lbm3d: the perturbation amplitude must be finite");
     }
@@ -360,6 +374,9 @@ int lbm3d_initialize_perturbed(lbm3d* h, uint64_t seed, double amplitude) {
 int lbm3d_step(lbm3d* h, uint64_t count) {
     if (h == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    if (h->in_batch) {
+        return refuse_in_batch("step");
     }
     return with_store(*h, [&]<class T>() -> int {
         const auto p = descriptor<T>(*h);
@@ -398,6 +415,9 @@ uint64_t lbm3d_steps(const lbm3d* h) { return h != nullptr ? h->steps : 0; }
 int lbm3d_set_inlet(lbm3d* h, const double velocity[3]) {
     if (h == nullptr || velocity == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    if (h->in_batch) {
+        return refuse_in_batch("set_inlet");
     }
     if (std::string p = lbm::host::inlet_problem3(h->cfg, velocity); !p.empty()) {
         return fail(LBM_EINVAL, p);
@@ -692,6 +712,9 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
     if (h == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null handle");
     }
+    if (h->in_batch) {
+        return refuse_in_batch("time_steps");
+    }
     const std::size_t marks = per_kernel ? 3U * count + 1U : 2U;
     std::vector<hipEvent_t> ev(marks, nullptr);
     int rc = LBM_OK;
@@ -807,6 +830,9 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
                    size_t* count, lbm_launch_choice* best) {
     if (h == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    if (h->in_batch) {
+        return refuse_in_batch("autotune");
     }
     if (steps == 0) {
         return fail(LBM_EINVAL, "lbm3d: autotune needs at least one timed step");

@@ -425,8 +425,8 @@ int rebuild(lbm_batch& b) {
             return rc;
         }
     }
-    b.dirty = false;
     if (b.entries.empty()) {
+        b.dirty = false;
         return ok();
     }
     hipError_t e = hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal);
@@ -446,7 +446,45 @@ int rebuild(lbm_batch& b) {
         return hip_fail_b("end capture", e);
     }
     e = hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0);
-    return e == hipSuccess ? ok() : hip_fail_b("graph instantiate", e);
+    if (e != hipSuccess) {
+        b.exec = nullptr;
+        return hip_fail_b("graph instantiate", e);
+    }
+    b.dirty = false; // only a complete capture clears it: any failure above rebuilds next run
+    return ok();
+}
+
+// Host side of the chunks [0, chunks) that ran: samples (in group order) into the lattices'
+// histories, then the step counts.
+void collect(lbm_batch& b, std::uint64_t chunks) {
+    const std::size_t n = b.entries.size();
+    for (std::uint64_t c = 0; c < chunks; ++c) {
+        std::size_t offset = 0;
+        for (lbm::Group& g : b.groups) {
+            for (std::size_t k = 0; k < g.members.size(); ++k) {
+                lbm_batch::Entry& entry = b.entries[static_cast<std::size_t>(g.members[k])];
+                const double* v = b.pinned + (c * n + offset + k) * 6;
+                entry.time.push_back(static_cast<double>(lbm3d_steps(entry.h) + (c + 1) * b.sample_every));
+                entry.force.insert(entry.force.end(), v, v + 6);
+            }
+            offset += g.members.size();
+        }
+    }
+    for (auto& entry : b.entries) {
+        lbm::internal::advance_steps(*entry.h, chunks * b.sample_every);
+    }
+}
+
+// Frees a lattice of the batch: the graph and the group tables (whose descriptors point into
+// the lattice) go first, after the stream has drained.
+void drop_entry(lbm_batch& b, std::vector<lbm_batch::Entry>::iterator it) {
+    (void)hipStreamSynchronize(b.stream);
+    release_groups(b);
+    b.used -= it->bytes;
+    it->h->in_batch = false;
+    lbm3d_destroy(it->h);
+    b.entries.erase(it);
+    b.dirty = true;
 }
 
 lbm_batch::Entry* find(lbm_batch& b, int id) {
@@ -502,6 +540,7 @@ int lbm_batch_destroy(lbm_batch* b) {
     (void)hipStreamSynchronize(b->stream);
     release_groups(*b);
     for (auto& e : b->entries) {
+        e.h->in_batch = false;
         lbm3d_destroy(e.h);
     }
     if (b->pinned != nullptr) {
@@ -549,6 +588,7 @@ int lbm_batch_add(lbm_batch* b, const lbm3d_config* config, const uint8_t* solid
             return rc;
         }
     }
+    h->in_batch = true;
     lbm_batch::Entry e;
     e.id = b->next_id++;
     e.h = h;
@@ -566,11 +606,7 @@ int lbm_batch_remove(lbm_batch* b, int32_t id) {
     }
     for (auto it = b->entries.begin(); it != b->entries.end(); ++it) {
         if (it->id == id) {
-            (void)hipStreamSynchronize(b->stream);
-            b->used -= it->bytes;
-            lbm3d_destroy(it->h);
-            b->entries.erase(it);
-            b->dirty = true;
+            drop_entry(*b, it);
             return ok();
         }
     }
@@ -612,8 +648,16 @@ int lbm_batch_run(lbm_batch* b, uint64_t steps) {
         }
         b->pinned_doubles = need;
     }
-    for (std::uint64_t c = 0; c < chunks; ++c) {
-        hipError_t e = hipGraphLaunch(b->exec, b->stream);
+    // A chunk is one graph replay plus the copies of its force samples; `queued` counts the
+    // chunks whose replay was enqueued (the device parity flips with each of them).
+    std::uint64_t queued = 0;
+    hipError_t e = hipSuccess;
+    for (std::uint64_t c = 0; c < chunks && e == hipSuccess; ++c) {
+        e = hipGraphLaunch(b->exec, b->stream);
+        if (e != hipSuccess) {
+            break;
+        }
+        ++queued;
         std::size_t offset = 0;
         for (lbm::Group& g : b->groups) {
             const std::size_t gn = g.members.size();
@@ -623,30 +667,175 @@ int lbm_batch_run(lbm_batch* b, uint64_t steps) {
             }
             offset += gn;
         }
-        if (e != hipSuccess) {
-            return hip_fail_b("graph launch", e);
+    }
+    const hipError_t drained = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess || drained != hipSuccess) {
+        // Keep the host step counts on the device parity: the replays that were enqueued ran
+        // (if the stream drained). The tables are uploaded again before the next run.
+        if (drained == hipSuccess) {
+            collect(*b, queued);
+        }
+        b->dirty = true;
+        return hip_fail_b(e != hipSuccess ? "graph launch" : "run", e != hipSuccess ? e : drained);
+    }
+    collect(*b, chunks);
+    return ok();
+}
+
+int lbm_batch_run_timed(lbm_batch* b, uint64_t steps, lbm_batch_timing* timing) {
+    if (b == nullptr || timing == nullptr) {
+        return fail(LBM_EINVAL, "lbm batch: null argument");
+    }
+    if (steps % b->sample_every != 0) {
+        return fail(LBM_EINVAL, "lbm batch: steps must be a multiple of the sample interval");
+    }
+    *timing = lbm_batch_timing{};
+    if (b->entries.empty() || steps == 0) {
+        return ok();
+    }
+    for (auto& e : b->entries) {
+        if (int rc = lbm3d_synchronize(e.h); rc != LBM_OK) {
+            return rc;
         }
     }
-    const hipError_t e = hipStreamSynchronize(b->stream);
-    if (e != hipSuccess) {
-        return hip_fail_b("run", e);
+    if (b->dirty) {
+        if (int rc = rebuild(*b); rc != LBM_OK) {
+            return rc;
+        }
     }
-    // Samples, in group order, back to the lattices; the step counts advance.
-    for (std::uint64_t c = 0; c < chunks; ++c) {
+    const std::uint64_t chunks = steps / b->sample_every;
+    const std::size_t n = b->entries.size();
+    const std::size_t need = chunks * n * 6;
+    if (need > b->pinned_doubles) {
+        if (b->pinned != nullptr) {
+            (void)hipHostFree(b->pinned);
+            b->pinned = nullptr;
+        }
+        if (hipHostMalloc(reinterpret_cast<void**>(&b->pinned), need * sizeof(double), hipHostMallocDefault) !=
+            hipSuccess) {
+            b->pinned_doubles = 0;
+            return fail(LBM_ENOMEM, "lbm batch: pinned sample buffer");
+        }
+        b->pinned_doubles = need;
+    }
+    // Marks: per step and group, before the bulk, before the boundary and after it; per chunk,
+    // before and after the force kernels; one at the very end.
+    const std::size_t groups = b->groups.size();
+    const std::size_t per_chunk = b->sample_every * groups * 3U + 2U;
+    std::vector<hipEvent_t> ev(chunks * per_chunk + 1, nullptr);
+    hipError_t e = hipSuccess;
+    for (auto& x : ev) {
+        if (e == hipSuccess) {
+            e = hipEventCreate(&x);
+        }
+    }
+    std::uint64_t queued = 0;
+    std::size_t m = 0;
+    for (std::uint64_t c = 0; c < chunks && e == hipSuccess; ++c) {
+        for (std::uint32_t s = 0; s < b->sample_every; ++s) {
+            for (lbm::Group& g : b->groups) {
+                const int st = g.storage;
+                (void)hipEventRecord(ev[m++], b->stream);
+                for_store(st, [&]<class T>() {
+                    using R = typename T::R;
+                    const auto* lat = static_cast<const lbm::Lattice3<R>*>(g.lat);
+                    const auto count = static_cast<std::uint32_t>(g.members.size());
+                    switch (g.cpt) {
+                        case 2:
+                            hipLaunchKernelGGL((lbm::batch_bulk<T, 2>), dim3(g.bulk_blocks), dim3(g.block), g.lds,
+                                               b->stream, lat, g.bulk_start, count, g.parity);
+                            break;
+                        case 4:
+                            hipLaunchKernelGGL((lbm::batch_bulk<T, 4>), dim3(g.bulk_blocks), dim3(g.block), g.lds,
+                                               b->stream, lat, g.bulk_start, count, g.parity);
+                            break;
+                        default:
+                            hipLaunchKernelGGL((lbm::batch_bulk<T, 1>), dim3(g.bulk_blocks), dim3(g.block), g.lds,
+                                               b->stream, lat, g.bulk_start, count, g.parity);
+                            break;
+                    }
+                    (void)hipEventRecord(ev[m++], b->stream);
+                    hipLaunchKernelGGL((lbm::batch_boundary<T>), dim3(g.bnd_blocks), dim3(256), 0, b->stream, lat,
+                                       g.bnd_start, count, g.parity);
+                    (void)hipEventRecord(ev[m++], b->stream);
+                    hipLaunchKernelGGL(lbm::batch_advance, dim3(1), dim3(256), 0, b->stream, g.parity, count);
+                });
+            }
+        }
+        (void)hipEventRecord(ev[m++], b->stream);
+        for (lbm::Group& g : b->groups) {
+            for_store(g.storage, [&]<class T>() { enqueue_force<T>(g, b->stream); });
+        }
+        (void)hipEventRecord(ev[m++], b->stream);
         std::size_t offset = 0;
         for (lbm::Group& g : b->groups) {
-            for (std::size_t k = 0; k < g.members.size(); ++k) {
-                lbm_batch::Entry& entry = b->entries[static_cast<std::size_t>(g.members[k])];
-                const double* v = b->pinned + (c * n + offset + k) * 6;
-                entry.time.push_back(static_cast<double>(lbm3d_steps(entry.h) + (c + 1) * b->sample_every));
-                entry.force.insert(entry.force.end(), v, v + 6);
+            const std::size_t gn = g.members.size();
+            if (e == hipSuccess) {
+                e = hipMemcpyAsync(b->pinned + (c * n + offset) * 6, g.force, gn * 48, hipMemcpyDeviceToHost,
+                                   b->stream);
             }
-            offset += g.members.size();
+            offset += gn;
+        }
+        if (e == hipSuccess) {
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) {
+            ++queued;
         }
     }
-    for (auto& entry : b->entries) {
-        lbm::internal::advance_steps(*entry.h, steps);
+    if (e == hipSuccess) {
+        e = hipEventRecord(ev[m], b->stream);
     }
+    const hipError_t drained = hipStreamSynchronize(b->stream);
+    if (e == hipSuccess && drained == hipSuccess) {
+        const auto span = [&](std::size_t i, std::size_t j) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[i], ev[j]);
+            return static_cast<double>(ms);
+        };
+        std::size_t k = 0;
+        double force_total = 0.0;
+        for (std::uint64_t c = 0; c < chunks; ++c) {
+            for (std::uint32_t s = 0; s < b->sample_every; ++s) {
+                for (lbm::Group& g : b->groups) {
+                    timing->bulk_ms[g.storage] += span(k, k + 1);
+                    timing->boundary_ms[g.storage] += span(k + 1, k + 2);
+                    k += 3;
+                }
+            }
+            force_total += span(k, k + 1);
+            k += 2;
+        }
+        timing->step_ms = span(0, m) / static_cast<double>(steps);
+        timing->force_ms = force_total / static_cast<double>(chunks);
+        for (int st = 0; st < 4; ++st) {
+            timing->bulk_ms[st] /= static_cast<double>(steps);
+            timing->boundary_ms[st] /= static_cast<double>(steps);
+        }
+        for (const lbm::Group& g : b->groups) {
+            for (const int member : g.members) {
+                std::size_t bulk = 0;
+                std::size_t boundary = 0;
+                (void)lbm3d_counts(b->entries[static_cast<std::size_t>(member)].h, &bulk, &boundary, nullptr,
+                                   nullptr);
+                timing->bulk_nodes[g.storage] += bulk;
+                timing->boundary_nodes[g.storage] += boundary;
+            }
+        }
+    }
+    for (auto& x : ev) {
+        if (x != nullptr) {
+            (void)hipEventDestroy(x);
+        }
+    }
+    if (e != hipSuccess || drained != hipSuccess) {
+        if (drained == hipSuccess) {
+            collect(*b, queued);
+        }
+        b->dirty = true;
+        return hip_fail_b("timed run", e != hipSuccess ? e : drained);
+    }
+    collect(*b, chunks);
     return ok();
 }
 
@@ -784,7 +973,9 @@ int lbm_batch_set_inlet(lbm_batch* b, int32_t id, const double velocity[3]) {
     if (e == nullptr) {
         return fail(LBM_EINVAL, "lbm batch: unknown lattice id");
     }
+    e->h->in_batch = false;
     const int rc = lbm3d_set_inlet(e->h, velocity);
+    e->h->in_batch = true;
     if (rc == LBM_OK) {
         b->dirty = true; // the descriptors carry the inlet velocity
     }

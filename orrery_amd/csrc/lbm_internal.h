@@ -63,6 +63,10 @@ struct lbm3d {
     std::vector<void*> owned;
     // Replayed runs of steps of lbm3d_step (serial and fused schedules).
     lbm::StepGraph graph;
+    // Held by a batch (lbm_batch_add): the batch steps the lattice on its own stream and keeps
+    // its A-A parity in device memory, so the calls that step, re-initialise, retarget or free
+    // the lattice refuse it; the queries and readbacks stay open.
+    bool in_batch = false;
 };
 
 namespace lbm::internal {

@@ -113,3 +113,46 @@ class CoefficientHistory:
             d += self.drag[k]
             l += self.lift[k]
         return Coefficients(d / n, l / n)
+
+
+def forces_settled(time, drag, lift, side, start: float, criteria: SettledCriteria = SettledCriteria()) -> bool:
+    """forces_settled_3d (src/aero/lbm3d/lbm_aero_solver3d.cpp:60-83): every force component is
+    steady or settled periodic shedding over the samples at time >= start."""
+    t, d, l, s = _arr(time), _arr(drag), _arr(lift), _arr(side)
+    n = min(len(t), len(d), len(l), len(s))
+    return bool(abi.load().lbm_forces_settled(abi.ptr(t, ctypes.c_double), abi.ptr(d, ctypes.c_double),
+                                              abi.ptr(l, ctypes.c_double), abi.ptr(s, ctypes.c_double), n, start,
+                                              criteria.steady_tolerance, criteria.max_period_spread,
+                                              criteria.stationarity))
+
+
+@dataclasses.dataclass
+class Verdict:
+    converged: bool
+    kind: str          # "steady", "settled" or "" (not converged)
+    drag: float        # mean drag coefficient over the judged samples
+    lift: float
+    start: float       # first sample time the means cover
+
+
+def judge_solve(time, drag, lift, side, min_time: float, window: int = 50, tolerance: float = 1e-4,
+                criteria: SettledCriteria = SettledCriteria()) -> Verdict:
+    """Convergence of a running solve from its coefficient history (one sample per entry).
+
+    Steady: the mean drag of the last `window` samples within `tolerance` of the mean of the
+    `window` before. Settled: forces_settled over the last half of the run (the reference's
+    adapter test, src/aero/lbm3d/lbm_aero_solver3d.cpp:564-584), so a wake that sheds at a
+    locked period with stationary amplitude and mean counts as converged. Neither is judged
+    before `min_time`. The means are over the last `window` samples (steady) or the last half of
+    the run (settled: whole shedding periods dominate it)."""
+    t, d, l, s = _arr(time), _arr(drag), _arr(lift), _arr(side)
+    n = min(len(t), len(d), len(l), len(s))
+    if n == 0 or t[n - 1] < min_time:
+        return Verdict(False, "", float("nan"), float("nan"), float("nan"))
+    tail = max(0, n - window)
+    if n >= 2 * window and abs(d[tail:n].mean() - d[n - 2 * window:tail].mean()) < tolerance:
+        return Verdict(True, "steady", float(d[tail:n].mean()), float(l[tail:n].mean()), float(t[tail]))
+    start = 0.5 * float(t[n - 1])
+    if forces_settled(t[:n], d[:n], l[:n], s[:n], start, criteria):
+        return Verdict(True, "settled", mean_since(t[:n], d[:n], start), mean_since(t[:n], l[:n], start), start)
+    return Verdict(False, "", float(d[tail:n].mean()), float(l[tail:n].mean()), float(t[tail]))

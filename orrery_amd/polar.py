@@ -3,9 +3,11 @@ or one lattice at a time: the batched Cd workload.
 
 Each case is an infinite-span section (the span runs through periodic y sides) of chord c
 cells in a channel with a velocity inlet, a convective outlet, slip z sides and an absorbing
-layer before the outlet, at its own Reynolds number and angle of attack. A case has converged
-when the mean drag coefficient of the last 50 samples (500 steps) differs from that of the 50
-before by less than 1e-4, after at least two flow-throughs. The lift slope is the least-squares
+layer before the outlet, at its own Reynolds number and angle of attack. A case has converged,
+after at least two flow-throughs, when the mean drag coefficient of the last 50 samples (500
+steps) differs from that of the 50 before by less than 1e-4 (a steady wake), or when the forces
+of the last half of the run are settled periodic shedding by the reference's adapter test
+(forces_settled_3d, src/aero/lbm3d/lbm_aero_solver3d.cpp:60-83). The lift slope is the least-squares
 slope of Cl against the angle of attack over the attached range, as in the reference's
 comparison of solvers (src/app/aero/aero_comparison.cpp:101-164).
 """
@@ -19,6 +21,7 @@ import numpy as np
 
 from . import autotune
 from . import geometry as geo
+from . import history
 from .batch import LatticeBatch
 from .config import Lbm3DConfig, Outlet, Smagorinsky, SolidWall, Sponge, Wall, WallKind, XBoundary
 from .masks import SolidMask3D, fluid_mask_3d
@@ -112,12 +115,16 @@ def coefficients(case: WingCase, force: np.ndarray) -> tuple[np.ndarray, np.ndar
     return force[:, 0] / q, force[:, 2] / q
 
 
-def converged(case: WingCase, time_steps: np.ndarray, cd: np.ndarray) -> bool:
-    if len(cd) < 2 * WINDOW or time_steps[-1] < case.min_steps:
-        return False
-    a = cd[-WINDOW:].mean()
-    b = cd[-2 * WINDOW:-WINDOW].mean()
-    return bool(abs(a - b) < TOLERANCE)
+def judge(case: WingCase, time_steps: np.ndarray, force: np.ndarray) -> history.Verdict:
+    """The convergence verdict of a case's force history (lattice forces [n, >=3]); the side
+    force is the spanwise (y) component."""
+    q = 0.5 * U_LATTICE * U_LATTICE * case.chord * case.ny
+    return history.judge_solve(time_steps, force[:, 0] / q, force[:, 2] / q, force[:, 1] / q, case.min_steps,
+                               WINDOW, TOLERANCE)
+
+
+def converged(case: WingCase, time_steps: np.ndarray, force: np.ndarray) -> bool:
+    return judge(case, time_steps, force).converged
 
 
 @dataclasses.dataclass
@@ -128,6 +135,7 @@ class Solve:
     cl: float
     converged: bool
     finished_at: float  # seconds from the start of the run
+    kind: str = ""      # "steady" or "settled" when converged
 
 
 def run_batched(cases: list[WingCase], lattices, storage: str = "f32", budget_bytes: int = 240 * 10 ** 9,
@@ -152,11 +160,10 @@ def run_batched(cases: list[WingCase], lattices, storage: str = "f32", budget_by
         for key in list(active):
             case, lat = active[key]
             t, f = lat.history()
-            cd, cl = coefficients(case, f)
-            ok = converged(case, t, cd)
-            if ok or t[-1] >= MAX_STEPS:
-                done.append(Solve(case, int(t[-1]), float(cd[-WINDOW:].mean()), float(cl[-WINDOW:].mean()), ok,
-                                  time.perf_counter() - t0))
+            v = judge(case, t, f)
+            if v.converged or t[-1] >= MAX_STEPS:
+                cd, cl = (v.drag, v.lift) if v.converged else [float(c[-WINDOW:].mean()) for c in coefficients(case, f)]
+                done.append(Solve(case, int(t[-1]), cd, cl, v.converged, time.perf_counter() - t0, v.kind))
                 batch.remove(lat)
                 del active[key]
                 if log:
@@ -183,11 +190,12 @@ def run_sequential(cases: list[WingCase], lattices, storage: str = "f32", seed: 
             forces.append(s.force().force)
             if len(times) % WINDOW == 0:
                 t = np.array(times)
-                cd, cl = coefficients(case, np.array(forces))
-                ok = converged(case, t, cd)
-                if ok or t[-1] >= MAX_STEPS:
-                    done.append(Solve(case, int(t[-1]), float(cd[-WINDOW:].mean()), float(cl[-WINDOW:].mean()), ok,
-                                      time.perf_counter() - t0))
+                f = np.array(forces)
+                v = judge(case, t, f)
+                if v.converged or t[-1] >= MAX_STEPS:
+                    cd, cl = (v.drag, v.lift) if v.converged else [float(c[-WINDOW:].mean())
+                                                                   for c in coefficients(case, f)]
+                    done.append(Solve(case, int(t[-1]), cd, cl, v.converged, time.perf_counter() - t0, v.kind))
                     if log:
                         log(f"sequential: case {case.index} done at step {int(t[-1])}")
                     break
@@ -233,4 +241,4 @@ def lift_slope(solves: list[Solve], attached=(-4.0, 8.0)) -> float:
 
 
 __all__ = ["WingCase", "wing_cases", "wing_lattice", "place_body", "run_batched", "run_sequential", "lift_slope",
-           "coefficients", "converged", "Solve", "records", "comparison_csv"]
+           "coefficients", "converged", "judge", "Solve", "records", "comparison_csv"]

@@ -5,8 +5,10 @@ goes in while a larger one waits), advanced together, and retired as each conver
 
 A request is a body (sphere, cone, cylinder, capsule or wedge: the reference's primitives,
 src/geom/primitives.cpp) at an angle of attack and Reynolds number in a channel of its own
-size, in binary32 or 16-bit storage. Converged: the mean drag coefficient of the last 50 samples
-(500 steps) within 1e-4 of the 50 before, after one flow-through.
+size, in binary32 or 16-bit storage. Converged, after one flow-through: the mean drag coefficient
+of the last 50 samples (500 steps) within 1e-4 of the 50 before (a steady wake), or the forces of
+the last half of the run settled periodic shedding by the reference's adapter test
+(forces_settled_3d, src/aero/lbm3d/lbm_aero_solver3d.cpp:60-83).
 """
 from __future__ import annotations
 
@@ -19,6 +21,7 @@ import numpy as np
 
 from . import autotune
 from . import geometry as geo
+from . import history
 from .batch import LatticeBatch
 from .config import Lbm3DConfig, Outlet, Smagorinsky, SolidWall, Sponge, Wall, WallKind, XBoundary
 from .polar import place_body
@@ -122,6 +125,14 @@ def drag_coefficient(req: Request, force: np.ndarray) -> np.ndarray:
     return force[:, 0] / (0.5 * U_LATTICE * U_LATTICE * req.length * req.length)
 
 
+def judge(req: Request, time_steps: np.ndarray, force: np.ndarray) -> history.Verdict:
+    """Convergence verdict of a request's force history: drag x, lift z, side force y, all over
+    the reference area L^2."""
+    q = 0.5 * U_LATTICE * U_LATTICE * req.length * req.length
+    return history.judge_solve(time_steps, force[:, 0] / q, force[:, 2] / q, force[:, 1] / q, req.min_steps,
+                               WINDOW, TOLERANCE)
+
+
 @dataclasses.dataclass
 class Served:
     request: Request
@@ -130,6 +141,8 @@ class Served:
     steps: int
     cd: float
     converged: bool
+    kind: str = ""          # "steady" or "settled" when converged
+    mean_from: float = 0.0  # first step of the samples the Cd mean covers
 
     @property
     def latency(self) -> float:
@@ -223,12 +236,12 @@ class Server:
                 for rid in list(active):
                     r, lat, admitted = active[rid]
                     t, f = lat.history()
+                    v = judge(r, t, f)
                     cd = drag_coefficient(r, f)
-                    ok = len(cd) >= 2 * WINDOW and t[-1] >= r.min_steps and \
-                        abs(cd[-WINDOW:].mean() - cd[-2 * WINDOW:-WINDOW].mean()) < TOLERANCE
-                    if ok or t[-1] >= MAX_STEPS or not np.isfinite(cd[-1]):
+                    if v.converged or t[-1] >= MAX_STEPS or not np.isfinite(cd[-1]):
                         done.append(Served(r, admitted, time.perf_counter() - t0, int(t[-1]),
-                                           float(cd[-WINDOW:].mean()), bool(ok)))
+                                           v.drag if v.converged else float(cd[-WINDOW:].mean()), v.converged,
+                                           v.kind, v.start))
                         batch.remove(lat)
                         del active[rid]
                         if log:
@@ -240,18 +253,23 @@ class Server:
         return done, time.perf_counter() - t0
 
 
-def fp64_reference_cd(req: Request, steps: int, seed: int = 5) -> float:
-    """The request's lattice alone in FP64 storage (the device FP64 oracle) to `steps`, Cd of
-    the last window."""
+def fp64_reference_cd(req: Request, steps: int, seed: int = 5, mean_from: float | None = None) -> float:
+    """The request's lattice alone in FP64 storage (the device FP64 oracle) to `steps`; Cd
+    averaged over the samples at step >= `mean_from` (default: the last window)."""
     config, mask = request_lattice(req)
     s = Lbm3DSolver(config, mask, "f64")
     s.initialize_perturbed(seed, 1e-4)
+    times = []
     forces = []
     while s.steps() < steps:
         s.step(SAMPLE_EVERY)
+        times.append(float(s.steps()))
         forces.append(s.force().force)
     s.close()
-    return float(drag_coefficient(req, np.array(forces[-WINDOW:])).mean())
+    cd = drag_coefficient(req, np.array(forces))
+    if mean_from is None:
+        return float(cd[-WINDOW:].mean())
+    return history.mean_since(np.array(times), cd, mean_from)
 
 
 def serve_bench(n: int, rate: float = 4.0, log=None, check: int = 3, order: str = "shortest",
@@ -269,7 +287,7 @@ def serve_bench(n: int, rate: float = 4.0, log=None, check: int = 3, order: str 
     errors = []
     pool = [s for s in served if s.converged] or served
     for s in sorted(pool, key=lambda s: s.request.cells)[:check]:
-        ref = fp64_reference_cd(s.request, s.steps)
+        ref = fp64_reference_cd(s.request, s.steps, mean_from=s.mean_from if s.converged else None)
         errors.append(abs(s.cd / ref - 1.0) if ref != 0 else abs(s.cd))
         if log:
             log(f"serve: request {s.request.id} {s.request.storage} Cd {s.cd:.6f} vs FP64 {ref:.6f}")

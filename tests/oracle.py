@@ -45,6 +45,8 @@ def load():
         "orc3d_mass": (ctypes.c_double, [ctypes.c_void_p]),
         "orc3d_macroscopic": (None, [ctypes.c_void_p, _f64p, _f64p, _f64p, _f64p]),
         "orc3d_populations": (None, [ctypes.c_void_p, _f64p]),
+        "orc3d_set_populations": (None, [ctypes.c_void_p, _f64p]),
+        "orc3d_outlet_history": (ctypes.c_size_t, [ctypes.c_void_p, _f64p]),
         "orc3d_links": (ctypes.c_size_t, [ctypes.c_void_p, _u32p, _u32p, _f64p, ctypes.c_size_t]),
         "orc3d_classes": (None, [ctypes.c_void_p, _u8p]),
         "orc3d_resolve": (ctypes.c_int, [P(abi.CConfig3D), _u8p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
@@ -161,6 +163,19 @@ class Oracle3D:
         f = np.empty(19 * self._config.cells)
         self._lib.orc3d_populations(self._h, ptr(f))
         return f
+
+    def set_populations(self, f):
+        """Replaces the state (logical layout f[i * cells + cell]); a test hook."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.size == 19 * self._config.cells
+        self._lib.orc3d_set_populations(self._h, ptr(f))
+
+    def outlet_history(self):
+        """The convective-outlet history, [5 slots][nz][ny] (slots: directions 2, 8, 10, 12, 14)."""
+        n = self._lib.orc3d_outlet_history(self._h, None)
+        out = np.empty(n)
+        self._lib.orc3d_outlet_history(self._h, ptr(out))
+        return out.reshape(5, self._config.nz, self._config.ny)
 
     def force(self) -> Force3D:
         f = np.zeros(3)

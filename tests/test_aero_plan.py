@@ -22,6 +22,10 @@ from orrery_amd.aero import (AeroError, AeroGeometry, FlowConditions, LbmAeroSol
 from orrery_amd.config import SolidWall, WallKind
 from orrery_amd.solver import boundary_links
 
+# Survey-recorded, not a pin: the adapter's coarse sphere (49x25x25, 320 steps) run through a
+# build of the reference whose oneTBB was replaced by a serial stand-in. The pins of the pieces
+# this run combines (inlet, convective outlet, absorbing layer, interpolated wall, force and
+# moment) are in tests/test_oracle_pins.py and tests/test_setup_pins.py.
 REFERENCE_CD = 3.5172643919827955
 
 

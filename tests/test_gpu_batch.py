@@ -94,3 +94,51 @@ def test_admission_refuses_beyond_the_budget():
         batch.add(c, m, "f32")
     used, budget = batch.bytes()
     assert used == 2 * need
+
+
+def test_a_batch_lattice_refuses_single_lattice_stepping():
+    """The batch keeps each lattice's A-A parity on the device; stepping, re-initialising,
+    retargeting or freeing its handle directly is refused (no device work is done), and the
+    lattice still matches its single twin afterwards."""
+    import ctypes
+    c, m, st, seed = lattices()[0]
+    batch = LatticeBatch(sample_every=10)
+    lat = batch.add(c, m, st, seed, 1e-4)
+    batch.run(20)
+    lib = batch._lib
+    h = lat._h()
+    v = (ctypes.c_double * 3)(0.05, 0.0, 0.0)
+    for name, call in (("step", lambda: lib.lbm3d_step(h, 2)), ("destroy", lambda: lib.lbm3d_destroy(h)),
+                       ("set_inlet", lambda: lib.lbm3d_set_inlet(h, v)),
+                       ("initialize", lambda: lib.lbm3d_initialize_perturbed(h, 1, 1e-4))):
+        assert call() != 0, name
+        assert b"belongs to a batch" in lib.lbm_last_error(), name
+    assert lat.steps() == 20
+    batch.run(20)
+    single, _ = alone(c, m, st, seed, 40)
+    assert np.array_equal(lat.populations(), single.populations())
+
+
+def test_timed_run_advances_exactly_as_a_run():
+    """The per-kernel timed run (eager launches between events) gives the same lattices and
+    force histories bit for bit as graph replays, and reports every storage group."""
+    steps = 40
+    plain = LatticeBatch(sample_every=10)
+    timed = LatticeBatch(sample_every=10)
+    pairs = []
+    for c, m, st, seed in lattices():
+        pairs.append((plain.add(c, m, st, seed, 1e-4), timed.add(c, m, st, seed, 1e-4)))
+    plain.run(steps)
+    report = timed.run_timed(20)
+    report2 = timed.run_timed(20)
+    assert set(report["groups"]) == {"f32", "f16s", "bf16s"}
+    for r in (report, report2):
+        assert r["step_ms"] > 0 and r["force_ms"] > 0
+        for g in r["groups"].values():
+            assert g["bulk_ms"] > 0 and g["boundary_ms"] > 0 and g["bulk_nodes"] > 0
+    for a, b in pairs:
+        assert b.steps() == steps
+        assert np.array_equal(a.populations(), b.populations())
+        ta, fa = a.history()
+        tb, fb = b.history()
+        assert np.array_equal(ta, tb) and np.array_equal(fa, fb)

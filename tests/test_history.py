@@ -115,3 +115,113 @@ def test_history_matches_the_reference_bitwise():
                 assert [o.period, o.frequency, float(o.cycles), o.amplitude, o.period_spread] == out.tolist()
             assert bool(ref.ref_steady_since(p(t), p(d), p(l), ctypes.c_size_t(len(t)), ctypes.c_double(start))) == \
                 steady_since(t, d, l, start)
+
+
+def random_histories(seed):
+    """Noisy, drifting, locked, steady and chirped signals (40-400 samples), with starts inside
+    and outside the history."""
+    rng = np.random.default_rng(seed)
+    for case in range(60):
+        n = int(rng.integers(40, 400))
+        t = np.cumsum(rng.choice([10.0, 10.0, 10.0, 5.0], size=n))
+        kind = case % 5
+        period = float(rng.uniform(60.0, 600.0))
+        phase = 2 * math.pi * t / period
+        if kind == 0:
+            lift = 0.3 * np.sin(phase) + 1e-3 * rng.standard_normal(n)
+        elif kind == 1:
+            lift = 0.3 * np.sin(phase * (1 + t / t[-1])) + 0.05 * rng.standard_normal(n)
+        elif kind == 2:
+            lift = 1e-6 * rng.standard_normal(n)
+        elif kind == 3:
+            lift = np.round(0.3 * np.sin(phase), 2)  # ties with the mean
+        else:
+            lift = np.cumsum(rng.standard_normal(n))
+        wobble = 1e-5 if kind == 2 else 0.01
+        drag = 1.2 + wobble * np.sin(2 * phase) + float(rng.uniform(0, 1e-3)) * wobble * rng.standard_normal(n)
+        side = (0.2 if case % 3 == 0 else 1e-7) * np.sin(phase + 1.0)
+        for start in (0.0, float(t[n // 2]), float(rng.uniform(t[0], t[-1])), float(t[-1]) + 1.0):
+            yield t, drag, lift, side, start
+
+
+@pytest.mark.skipif(not refcheck.available(), reason="oracle/_ref not built (reference tree absent)")
+def test_history_matches_the_reference_on_random_signals():
+    ref = refcheck.load()
+    for name in ("ref_dominant_oscillation", "ref_settled_oscillation", "ref_steady_since"):
+        getattr(ref, name).restype = ctypes.c_int
+    ref.ref_mean_since.restype = ctypes.c_double
+    p = refcheck.ptr
+    out = np.zeros(5)
+    seen = {"dominant": 0, "settled": 0, "steady": 0}
+    for t, d, l, s, start in random_histories(11):
+        n = ctypes.c_size_t(len(t))
+        c = ctypes.c_double(start)
+        assert ref.ref_mean_since(p(t), p(l), n, c) == mean_since(t, l, start)
+        ok = ref.ref_dominant_oscillation(p(t), p(l), n, c, p(out))
+        o = dominant_oscillation(t, l, start)
+        assert bool(ok) == (o is not None)
+        if ok:
+            seen["dominant"] += 1
+            assert [o.period, o.frequency, float(o.cycles), o.amplitude, o.period_spread] == out.tolist()
+        ok = ref.ref_settled_oscillation(p(t), p(d), p(l), n, c, p(out))
+        o = settled_oscillation(t, d, l, start)
+        assert bool(ok) == (o is not None)
+        if ok:
+            seen["settled"] += 1
+            assert [o.period, o.frequency, float(o.cycles), o.amplitude, o.period_spread] == out.tolist()
+        ok = bool(ref.ref_steady_since(p(t), p(d), p(l), n, c))
+        seen["steady"] += ok
+        assert ok == steady_since(t, d, l, start)
+    # every branch was exercised
+    assert min(seen.values()) > 0, seen
+
+
+def forces_settled_from_reference(ref, t, d, l, s, start):
+    """forces_settled_3d (lbm_aero_solver3d.cpp:60-83) composed from the reference's own
+    mean_since and settled_oscillation (oracle/_ref): per component, two or more samples at
+    time >= start within 1e-3 x |mean drag|, or settled shedding paired with the drag."""
+    p = refcheck.ptr
+    n = ctypes.c_size_t(len(t))
+    bound = 1e-3 * abs(ref.ref_mean_since(p(t), p(d), n, ctypes.c_double(start)))
+    out = np.zeros(5)
+    for v in (d, l, s):
+        kept = v[t >= start]
+        if len(kept) >= 2 and kept.max() - kept.min() <= bound:
+            continue
+        if not ref.ref_settled_oscillation(p(t), p(d), p(v), n, ctypes.c_double(start), p(out)):
+            return False
+    return True
+
+
+@pytest.mark.skipif(not refcheck.available(), reason="oracle/_ref not built (reference tree absent)")
+def test_forces_settled_matches_the_reference_composition():
+    from orrery_amd.history import forces_settled
+    ref = refcheck.load()
+    ref.ref_settled_oscillation.restype = ctypes.c_int
+    ref.ref_mean_since.restype = ctypes.c_double
+    verdicts = set()
+    for t, d, l, s, start in random_histories(5):
+        want = forces_settled_from_reference(ref, t, d, l, s, start)
+        verdicts.add(want)
+        assert forces_settled(t, d, l, s, start) == want
+    assert verdicts == {True, False}
+
+
+def test_forces_settled_accepts_settled_shedding():
+    """A wake shedding at a locked period with a steady side force counts as converged, as the
+    reference's forces_settled_3d has it; a growing or drifting one does not."""
+    from orrery_amd.history import forces_settled
+    cases = settled_cases()
+    t, d, l = cases["locked"]
+    side = np.full_like(t, 1e-9)
+    assert forces_settled(t, d, l, side, 10000.0)
+    assert not steady_since(t, d, l, 10000.0)
+    for name in ("growing", "uneven", "drifting"):
+        t, d, l = cases[name]
+        assert not forces_settled(t, d, l, side, 10000.0), name
+    # shedding in the side force too (a wake rotating its plane) is still settled
+    t, d, l = cases["locked"]
+    assert forces_settled(t, d, l, l[::-1].copy(), 10000.0)
+    # steady flow: all three within 1e-3 of the drag
+    steady = np.full_like(t, 1.4)
+    assert forces_settled(t, steady, 1e-6 * np.ones_like(t), side, 10000.0)
