@@ -150,6 +150,11 @@ int lbm3d_synchronize(lbm3d* h);
  * (1, 2 or 4 consecutive cells, each population plane moved as one access). 0 keeps the
  * current value. Used by the autotuner. */
 int lbm3d_set_launch(lbm3d* h, int32_t block, int32_t cells_per_thread);
+/* Cells per thread of the bulk kernel for the A step (odd step count) and the B step (even)
+ * separately (1, 2 or 4; 0 keeps the current value); lbm3d_set_launch sets both. No result
+ * changes. */
+int lbm3d_set_cells_per_thread(lbm3d* h, int32_t a_step, int32_t b_step);
+int lbm3d_get_cells_per_thread(const lbm3d* h, int32_t* a_step, int32_t* b_step);
 /* How a step is launched: LBM_SCHEDULE_SERIAL (default) runs the bulk kernel then the
  * boundary kernel on the handle's stream; LBM_SCHEDULE_STREAMS runs the two concurrently on
  * two streams; LBM_SCHEDULE_FUSED runs bulk and boundary nodes in one kernel in cell order.
@@ -204,6 +209,9 @@ int lbm3d_counts(const lbm3d* h, size_t* bulk, size_t* boundary, size_t* solid, 
  * brackets every launch, giving the summed durations of the bulk and boundary kernels. */
 int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_ms, double* bulk_ms,
                      double* boundary_ms);
+/* Advances `count` steps with HIP events around each kernel on the handle's stream: ms[0], ms[1]
+ * the bulk kernel's mean duration in A and B steps, ms[2], ms[3] the boundary kernel's. */
+int lbm3d_time_kernels(lbm3d* h, uint64_t count, double ms[4]);
 
 /* ---- D2Q9 (same shape; lbm2d.h:173-211, lbm2d_gpu.cpp:68-378) ---------------------------- */
 int lbm2d_create(const lbm2d_config* config, const uint8_t* solid, int32_t storage, lbm2d** out);

@@ -153,6 +153,9 @@ EXPORTS = [
     ("lbm3d_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, P(CLaunchChoice),
                                       ctypes.c_size_t, c_sizep, P(CLaunchChoice)]),
     ("lbm3d_set_launch", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    ("lbm3d_set_cells_per_thread", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    ("lbm3d_get_cells_per_thread", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int32), P(ctypes.c_int32)]),
+    ("lbm3d_time_kernels", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, c_f64p]),
     ("lbm3d_set_occupancy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     ("lbm3d_get_occupancy", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int32), P(ctypes.c_int32)]),
     ("lbm3d_step_bytes", ctypes.c_int, [ctypes.c_void_p, c_f64p]),

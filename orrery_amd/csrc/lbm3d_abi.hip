@@ -111,12 +111,12 @@ template <class T> hipError_t step_once(lbm3d& h, const lbm::Lattice3<typename T
         return lbm::d3q19_step_fused<T>(p, local, h.stream);
     }
     if (h.schedule == LBM_SCHEDULE_SERIAL || p.nb == 0) {
-        return lbm::d3q19_step<T>(p, local, h.block, h.cpt, bulk_lds(h, local), h.stream);
+        return lbm::d3q19_step<T>(p, local, h.block, local ? h.cpt : h.cpt_b, bulk_lds(h, local), h.stream);
     }
     hipError_t e = hipEventRecord(h.fork, h.stream);
     if (e == hipSuccess) e = hipStreamWaitEvent(h.side, h.fork, 0);
     if (e == hipSuccess) e = lbm::d3q19_step_boundary<T>(p, local, h.side);
-    if (e == hipSuccess) e = lbm::d3q19_step_bulk<T>(p, local, h.block, h.cpt, bulk_lds(h, local), h.stream);
+    if (e == hipSuccess) e = lbm::d3q19_step_bulk<T>(p, local, h.block, local ? h.cpt : h.cpt_b, bulk_lds(h, local), h.stream);
     if (e == hipSuccess) e = hipEventRecord(h.join, h.side);
     if (e == hipSuccess) e = hipStreamWaitEvent(h.stream, h.join, 0);
     return e;
@@ -191,7 +191,7 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
         return fail(LBM_EINVAL, p);
     }
     const std::size_t cells = c.nx * c.ny * c.nz;
-    const std::uint32_t stride = lbm::host::padded(cells, 128);
+    const std::uint32_t stride = lbm::internal::plane_stride(cells);
     // Every index the kernels form (population i * stride + cell) must fit in 32 bits
     // (lbm3d_gpu.cpp:492-497).
     if (static_cast<std::uint64_t>(stride) * 19U + 2U * lbm::internal::guard_cells(c.nx, c.ny) >
@@ -210,6 +210,7 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     h->cfg = c;
     h->storage = storage;
     h->cpt = lbm::internal::default_cells_per_thread(storage);
+    h->cpt_b = h->cpt;
     h->cells = cells;
     h->stride = stride;
     h->solid.assign(solid, solid + cells);
@@ -396,6 +397,7 @@ int lbm3d_step(lbm3d* h, uint64_t count) {
         lbm::key_append(key, h->schedule);
         lbm::key_append(key, h->block);
         lbm::key_append(key, h->cpt);
+        lbm::key_append(key, h->cpt_b);
         lbm::key_append(key, h->groups);
         const hipError_t e = h->graph.run(h->stream, count, h->steps, key, launch);
         return e == hipSuccess ? ok() : hip_fail("step", e);
@@ -669,7 +671,33 @@ int lbm3d_set_launch(lbm3d* h, int32_t block, int32_t cells_per_thread) {
         return fail(LBM_EINVAL, "lbm3d: cells per thread must be 1, 2 or 4");
     }
     if (block != 0) h->block = block;
-    if (cells_per_thread != 0) h->cpt = cells_per_thread;
+    if (cells_per_thread != 0) {
+        h->cpt = cells_per_thread;
+        h->cpt_b = cells_per_thread;
+    }
+    return ok();
+}
+
+int lbm3d_set_cells_per_thread(lbm3d* h, int32_t a_step, int32_t b_step) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    for (const int32_t v : {a_step, b_step}) {
+        if (v != 0 && v != 1 && v != 2 && v != 4) {
+            return fail(LBM_EINVAL, "lbm3d: cells per thread must be 1, 2 or 4");
+        }
+    }
+    if (a_step != 0) h->cpt = a_step;
+    if (b_step != 0) h->cpt_b = b_step;
+    return ok();
+}
+
+int lbm3d_get_cells_per_thread(const lbm3d* h, int32_t* a_step, int32_t* b_step) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    if (a_step != nullptr) *a_step = h->cpt;
+    if (b_step != nullptr) *b_step = h->cpt_b;
     return ok();
 }
 
@@ -734,7 +762,8 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
                     const hipError_t err = step_once<T>(*h, p, local);
                     return err == hipSuccess ? ok() : hip_fail("timed step", err);
                 }
-                hipError_t err = lbm::d3q19_step_bulk<T>(p, local, h->block, h->cpt, bulk_lds(*h, local), h->stream);
+                hipError_t err = lbm::d3q19_step_bulk<T>(p, local, h->block, local ? h->cpt : h->cpt_b,
+                                                         bulk_lds(*h, local), h->stream);
                 if (err == hipSuccess && per_kernel) {
                     err = hipEventRecord(ev[3U * n + 1U], h->stream);
                 }
@@ -778,6 +807,73 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
             }
             if (bulk_ms != nullptr) *bulk_ms = bulk;
             if (boundary_ms != nullptr) *boundary_ms = bnd;
+        }
+    }
+    for (auto& e : ev) {
+        if (e != nullptr) {
+            (void)hipEventDestroy(e);
+        }
+    }
+    return rc;
+}
+
+int lbm3d_time_kernels(lbm3d* h, uint64_t count, double ms[4]) {
+    if (h == nullptr || ms == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    if (h->in_batch) {
+        return refuse_in_batch("time_kernels");
+    }
+    // Marks around each kernel: [bulk start, bulk end = boundary start, boundary end] per step.
+    std::vector<hipEvent_t> ev(3U * count, nullptr);
+    int rc = LBM_OK;
+    for (auto& e : ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            rc = fail(LBM_EDEVICE, "lbm3d gpu: event creation failed");
+            break;
+        }
+    }
+    for (std::uint64_t n = 0; n < count && rc == LBM_OK; ++n) {
+        const bool local = (h->steps & 1U) != 0;
+        rc = with_store(*h, [&]<class T>() -> int {
+            const auto p = descriptor<T>(*h);
+            hipError_t err = hipEventRecord(ev[3U * n], h->stream);
+            if (err == hipSuccess) {
+                err = lbm::d3q19_step_bulk<T>(p, local, h->block, local ? h->cpt : h->cpt_b, bulk_lds(*h, local),
+                                              h->stream);
+            }
+            if (err == hipSuccess) err = hipEventRecord(ev[3U * n + 1U], h->stream);
+            if (err == hipSuccess) err = lbm::d3q19_step_boundary<T>(p, local, h->stream);
+            if (err == hipSuccess) err = hipEventRecord(ev[3U * n + 2U], h->stream);
+            return err == hipSuccess ? ok() : hip_fail("timed kernels", err);
+        });
+        if (rc == LBM_OK) {
+            ++h->steps;
+        }
+    }
+    if (rc == LBM_OK && count > 0) {
+        const hipError_t e = hipEventSynchronize(ev[3U * count - 1U]);
+        if (e != hipSuccess) {
+            rc = hip_fail("timed kernels", e);
+        }
+    }
+    if (rc == LBM_OK) {
+        double sum[4] = {0, 0, 0, 0};
+        std::uint64_t launches[2] = {0, 0};
+        std::uint64_t first = h->steps - count;
+        for (std::uint64_t n = 0; n < count; ++n) {
+            const int b_step = ((first + n) & 1U) == 0 ? 1 : 0; // even step count before: a B step
+            float a = 0.0f;
+            float b = 0.0f;
+            (void)hipEventElapsedTime(&a, ev[3U * n], ev[3U * n + 1U]);
+            (void)hipEventElapsedTime(&b, ev[3U * n + 1U], ev[3U * n + 2U]);
+            sum[b_step] += a;
+            sum[2 + b_step] += b;
+            ++launches[b_step];
+        }
+        for (int k = 0; k < 4; ++k) {
+            const std::uint64_t l = launches[k & 1];
+            ms[k] = l > 0 ? sum[k] / static_cast<double>(l) : 0.0;
         }
     }
     for (auto& e : ev) {
@@ -864,6 +960,7 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
     const int saved_schedule = h->schedule;
     const int saved_block = h->block;
     const int saved_cpt = h->cpt;
+    const int saved_cpt_b = h->cpt_b;
     const int saved_groups[2] = {h->groups[0], h->groups[1]};
     const std::uint64_t saved_steps = h->steps;
     std::vector<lbm_launch_choice> cand;
@@ -880,6 +977,7 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
         h->schedule = c.schedule;
         h->block = c.block;
         h->cpt = c.cells_per_thread;
+        h->cpt_b = c.cells_per_thread;
         h->groups[0] = c.groups_a;
         h->groups[1] = c.groups_b;
         double ms = 0.0;
@@ -982,6 +1080,7 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
         h->schedule = saved_schedule;
         h->block = saved_block;
         h->cpt = saved_cpt;
+        h->cpt_b = saved_cpt_b;
         h->groups[0] = saved_groups[0];
         h->groups[1] = saved_groups[1];
         return rc;
@@ -990,6 +1089,7 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
     h->schedule = cand[k].schedule;
     h->block = cand[k].block;
     h->cpt = cand[k].cells_per_thread;
+    h->cpt_b = cand[k].cells_per_thread;
     h->groups[0] = cand[k].groups_a;
     h->groups[1] = cand[k].groups_b;
     if (best != nullptr) *best = cand[k];

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -37,7 +38,8 @@ struct lbm3d {
     std::uint64_t steps = 0;
     double inlet[3] = {0, 0, 0};
     int block = 256;
-    int cpt = 1;
+    int cpt = 1;    // cells per thread of the bulk kernel, A step
+    int cpt_b = 1;  // and B step (its x-shifted planes may prefer another width)
     // Cap on the bulk kernel's workgroups per compute unit, A step and B step (0: none), and
     // the device's LDS per compute unit and per workgroup that the cap is reserved from.
     int groups[2] = {0, 0};
@@ -91,6 +93,18 @@ inline int default_cells_per_thread(int storage) {
     return storage == LBM_STORE_F16S || storage == LBM_STORE_BF16S ? 2 : 1;
 }
 
+// Elements between the starts of consecutive population planes: the cells rounded up to 128,
+// plus ORRERY_LBM_PLANE_PAD elements (a multiple of 128, default 0) read from the environment
+// once, for layout experiments.
+inline std::uint32_t plane_stride(std::size_t cells) {
+    static const std::uint32_t pad = [] {
+        const char* env = std::getenv("ORRERY_LBM_PLANE_PAD");
+        const long v = env != nullptr ? std::strtol(env, nullptr, 10) : 0;
+        return v > 0 && v < (1L << 24) ? static_cast<std::uint32_t>(v) / 128U * 128U : 0U;
+    }();
+    return lbm::host::padded(cells, 128) + pad;
+}
+
 // Guard cells before and after the population planes: the B step of the bulk kernel loads the
 // sources of every cell of its tile before it knows whether the cell is a bulk node, and those
 // sources lie at most nx ny + nx + 1 cells away. Rounded to 128 cells to keep the planes aligned.
@@ -125,7 +139,7 @@ inline std::size_t required_bytes3(const lbm3d_config& c, const std::uint8_t* so
                                    int storage) {
     const std::size_t cells = c.nx * c.ny * c.nz;
     const lbm::host::Tables t = lbm::host::tables3(c, solid, distance);
-    return lattice_bytes(cells, lbm::host::padded(cells, 128), guard_cells(c.nx, c.ny), t, storage,
+    return lattice_bytes(cells, plane_stride(cells), guard_cells(c.nx, c.ny), t, storage,
                          c.solid_wall == LBM_SOLID_INTERPOLATED);
 }
 

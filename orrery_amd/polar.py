@@ -91,7 +91,9 @@ def place_body(mesh: geo.TriangleMesh, nx: int, ny: int, nz: int, interpolated: 
     return mask
 
 
-def wing_lattice(case: WingCase, interpolated: bool = True) -> tuple[Lbm3DConfig, SolidMask3D]:
+def wing_lattice(case: WingCase, interpolated: bool = True, device: bool = False) -> tuple[Lbm3DConfig, SolidMask3D]:
+    """The case's lattice; with `device` the wing is voxelised on the GPU (the same mask and
+    distances bit for bit)."""
     c = case.chord
     # A section longer than the periodic span, so no cell sees the end caps within the
     # three-cell distance band.
@@ -99,7 +101,7 @@ def wing_lattice(case: WingCase, interpolated: bool = True) -> tuple[Lbm3DConfig
     pivot = (0.25 * c, 0.0, 0.0)
     to = (round(0.3 * case.nx) + 0.25, 0.5 * (case.ny - 1), 0.5 * (case.nz - 1) + 0.25)
     mesh = wing.transformed(geo.rotation_y(math.radians(case.aoa_deg)), pivot, to)
-    mask = place_body(mesh, case.nx, case.ny, case.nz, interpolated)
+    mask = place_body(mesh, case.nx, case.ny, case.nz, interpolated, device=device)
     tau = 0.5 + 3.0 * U_LATTICE * c / case.reynolds
     config = Lbm3DConfig(nx=case.nx, ny=case.ny, nz=case.nz, tau=tau, x_boundary=XBoundary.inflow_outflow,
                          outlet=Outlet.convective, y_low=Wall(WallKind.periodic), y_high=Wall(WallKind.periodic),

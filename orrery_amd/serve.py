@@ -167,7 +167,7 @@ class Server:
 
     def __init__(self, budget_bytes: int = 240 * 10 ** 9, chunk: int = WINDOW * SAMPLE_EVERY, workers: int = 4,
                  seed: int = 5, tune_steps: int = 10, max_active_cells: int | None = 48_000_000,
-                 order: str = "shortest", max_wait: float = 30.0):
+                 order: str = "shortest", max_wait: float = 30.0, max_steps: int = MAX_STEPS):
         self.budget = budget_bytes
         self.chunk = chunk
         self.workers = workers
@@ -176,6 +176,7 @@ class Server:
         self.max_active_cells = max_active_cells
         self.order = order
         self.max_wait = max_wait
+        self.max_steps = max_steps  # a request leaves unconverged at this many steps
         self.busy = 0.0  # seconds inside batch.run
 
     def run(self, reqs: list[Request], log=None) -> tuple[list[Served], float]:
@@ -238,7 +239,7 @@ class Server:
                     t, f = lat.history()
                     v = judge(r, t, f)
                     cd = drag_coefficient(r, f)
-                    if v.converged or t[-1] >= MAX_STEPS or not np.isfinite(cd[-1]):
+                    if v.converged or t[-1] >= self.max_steps or not np.isfinite(cd[-1]):
                         done.append(Served(r, admitted, time.perf_counter() - t0, int(t[-1]),
                                            v.drag if v.converged else float(cd[-WINDOW:].mean()), v.converged,
                                            v.kind, v.start))
