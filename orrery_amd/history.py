@@ -136,15 +136,19 @@ class Verdict:
 
 
 def judge_solve(time, drag, lift, side, min_time: float, window: int = 50, tolerance: float = 1e-4,
-                criteria: SettledCriteria = SettledCriteria()) -> Verdict:
-    """Convergence of a running solve from its coefficient history (one sample per entry).
+                criteria: SettledCriteria = SettledCriteria(), stationarity: float = 5e-3) -> Verdict:
+    """Convergence of a running solve from its coefficient history (one sample per entry), by
+    the first of three tests that holds, none judged before `min_time`:
 
-    Steady: the mean drag of the last `window` samples within `tolerance` of the mean of the
-    `window` before. Settled: forces_settled over the last half of the run (the reference's
-    adapter test, src/aero/lbm3d/lbm_aero_solver3d.cpp:564-584), so a wake that sheds at a
-    locked period with stationary amplitude and mean counts as converged. Neither is judged
-    before `min_time`. The means are over the last `window` samples (steady) or the last half of
-    the run (settled: whole shedding periods dominate it)."""
+    - steady: the mean drag of the last `window` samples within `tolerance` of the mean of the
+      `window` before; the means cover the last window;
+    - settled: forces_settled over the last half of the run (the reference's adapter test,
+      src/aero/lbm3d/lbm_aero_solver3d.cpp:564-584): a wake shedding at a locked period with
+      stationary amplitude and mean; the means cover the last half;
+    - stationary (no reference counterpart; for irregular wakes that neither test accepts):
+      the last half of the run split into two halves of equal sample count, whose mean drag
+      agree within `stationarity` x |mean drag| and whose mean lift agree within
+      `stationarity` x max(|mean drag|, |mean lift|); the means cover the last half."""
     t, d, l, s = _arr(time), _arr(drag), _arr(lift), _arr(side)
     n = min(len(t), len(d), len(l), len(s))
     if n == 0 or t[n - 1] < min_time:
@@ -155,4 +159,12 @@ def judge_solve(time, drag, lift, side, min_time: float, window: int = 50, toler
     start = 0.5 * float(t[n - 1])
     if forces_settled(t[:n], d[:n], l[:n], s[:n], start, criteria):
         return Verdict(True, "settled", mean_since(t[:n], d[:n], start), mean_since(t[:n], l[:n], start), start)
+    half = n // 2
+    if n - half >= 2 * window:
+        mid = half + (n - half) // 2
+        cd, cl = float(d[half:n].mean()), float(l[half:n].mean())
+        scale = max(abs(cd), abs(cl))
+        if abs(d[half:mid].mean() - d[mid:n].mean()) <= stationarity * abs(cd) and \
+                abs(l[half:mid].mean() - l[mid:n].mean()) <= stationarity * scale:
+            return Verdict(True, "stationary", cd, cl, float(t[half]))
     return Verdict(False, "", float(d[tail:n].mean()), float(l[tail:n].mean()), float(t[tail]))

@@ -7,7 +7,9 @@ layer before the outlet, at its own Reynolds number and angle of attack. A case 
 after at least two flow-throughs, when the mean drag coefficient of the last 50 samples (500
 steps) differs from that of the 50 before by less than 1e-4 (a steady wake), or when the forces
 of the last half of the run are settled periodic shedding by the reference's adapter test
-(forces_settled_3d, src/aero/lbm3d/lbm_aero_solver3d.cpp:60-83). The lift slope is the least-squares
+(forces_settled_3d, src/aero/lbm3d/lbm_aero_solver3d.cpp:60-83), or when the mean coefficients of
+the two quarters that make up the last half of the run agree within 0.5% (an irregular wake
+whose time average has settled; history.judge_solve). The lift slope is the least-squares
 slope of Cl against the angle of attack over the attached range, as in the reference's
 comparison of solvers (src/app/aero/aero_comparison.cpp:101-164).
 """

@@ -8,7 +8,9 @@ src/geom/primitives.cpp) at an angle of attack and Reynolds number in a channel 
 size, in binary32 or 16-bit storage. Converged, after one flow-through: the mean drag coefficient
 of the last 50 samples (500 steps) within 1e-4 of the 50 before (a steady wake), or the forces of
 the last half of the run settled periodic shedding by the reference's adapter test
-(forces_settled_3d, src/aero/lbm3d/lbm_aero_solver3d.cpp:60-83).
+(forces_settled_3d, src/aero/lbm3d/lbm_aero_solver3d.cpp:60-83), or the mean coefficients of the two
+quarters that make up the last half of the run within 0.5% of each other (an irregular wake whose
+time average has settled; history.judge_solve).
 """
 from __future__ import annotations
 

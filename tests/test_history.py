@@ -225,3 +225,28 @@ def test_forces_settled_accepts_settled_shedding():
     # steady flow: all three within 1e-3 of the drag
     steady = np.full_like(t, 1.4)
     assert forces_settled(t, steady, 1e-6 * np.ones_like(t), side, 10000.0)
+
+
+def test_judge_solve_kinds():
+    """The three convergence kinds of a running solve, on synthetic histories sampled every
+    10 steps: a decaying drag (steady), locked shedding (settled), an irregular wake about a
+    settled mean (stationary), and a drifting one (not converged)."""
+    from orrery_amd.history import judge_solve
+    t = np.arange(10.0, 20010.0, 10.0)
+    zero = np.zeros_like(t)
+    steady = 1.2 + 0.5 * np.exp(-t / 800.0)
+    v = judge_solve(t, steady, 1e-9 * np.ones_like(t), zero, 4000.0)
+    assert v.converged and v.kind == "steady" and v.drag == pytest.approx(1.2, rel=1e-4)
+    assert not judge_solve(t[t < 3000], steady[t < 3000], zero[t < 3000], zero[t < 3000], 4000.0).converged
+    d = 1.4 + 0.01 * np.sin(2 * np.pi * t / 350.0)  # no whole number of periods in a 500-step window
+    l = 0.4 * np.sin(2 * np.pi * t / 700.0)
+    v = judge_solve(t, d, l, zero, 4000.0)
+    assert v.converged and v.kind == "settled" and v.drag == pytest.approx(1.4, rel=1e-3)
+    rng = np.random.default_rng(1)
+    noise = np.convolve(rng.standard_normal(len(t) + 20), np.ones(20) / 20, mode="valid")[:len(t)]
+    irregular = 0.9 + 0.02 * noise + 0.01 * np.sin(t / 37.0) * np.sin(t / 151.0)
+    v = judge_solve(t, irregular, 0.3 + 0.02 * noise[::-1], zero, 4000.0)
+    assert v.converged and v.kind == "stationary"
+    assert v.drag == pytest.approx(irregular[len(t) // 2:].mean(), rel=1e-12)
+    drifting = irregular + 2e-6 * t
+    assert not judge_solve(t, drifting, 0.3 + 0.02 * noise[::-1], zero, 4000.0).converged
