@@ -134,6 +134,15 @@ int lbm3d_macroscopic(lbm3d* h, double* rho, double* ux, double* uy, double* uz)
  * instead of 4 cells (the adapter's wall-pressure samples and inlet-plane density). Any output
  * may be NULL; values are those lbm3d_macroscopic reports at the same nodes. */
 int lbm3d_sample(lbm3d* h, const uint32_t* cells, size_t n, double* rho, double* ux, double* uy, double* uz);
+/* lbm3d_sample without allocations. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (serving loops, harnesses that forbid hipMalloc). Its node list,
+ * boundary slots and outputs live in a fixed device array of the library (2.5 MiB per device,
+ * held until the process ends) instead of a per-call buffer; at most 65536 nodes per call
+ * (LBM_EINVAL beyond, before any device work). Same values as lbm3d_sample bit for bit. Calls
+ * on one device share its array: they are serialised within the process, and return once the
+ * outputs are on the host. */
+int lbm3d_sample_fixed(lbm3d* h, const uint32_t* cells, size_t n, double* rho, double* ux, double* uy,
+                       double* uz);
 /* max_speed(macroscopic()) (lbm3d_common.cpp:200-211), reduced on the device. */
 int lbm3d_max_speed(lbm3d* h, double* speed);
 /* Lbm3DSolver::populations (lbm3d_gpu.cpp:394-405): 19 * cells doubles, logical layout. */

@@ -142,6 +142,8 @@ EXPORTS = [
     ("lbm3d_macroscopic", ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, c_f64p, c_f64p]),
     ("lbm3d_sample", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint32), ctypes.c_size_t, c_f64p, c_f64p, c_f64p,
                                     c_f64p]),
+    ("lbm3d_sample_fixed", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_uint32), ctypes.c_size_t, c_f64p, c_f64p,
+                                          c_f64p, c_f64p]),
     ("lbm3d_max_speed", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_populations", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_links", ctypes.c_int, [ctypes.c_void_p, c_u32p, c_u32p, c_f64p, ctypes.c_size_t, c_sizep]),

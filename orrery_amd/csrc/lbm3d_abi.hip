@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -580,6 +581,126 @@ int lbm3d_sample(lbm3d* h, const uint32_t* cells, size_t n, double* rho, double*
     }
     const int sy = sync(*h, "sample readback");
     (void)hipFree(buffer);
+    return rc != LBM_OK ? rc : sy;
+}
+
+} // extern "C"
+
+// ---- lbm3d_sample_fixed: lbm3d_sample without allocations -----------------------------------
+// The per-call buffer of lbm3d_sample (node list, boundary slots, four output planes) lives in
+// one file-scope device array instead. Each device holds its own copy; its address on the
+// current device is looked up once (hipGetSymbolAddress) and kept per device. The host names
+// `scratch` only as a symbol handle: kernels, copies and fills take the looked-up address.
+namespace {
+
+constexpr std::size_t kSampleMax = 1U << 16;  // nodes per call
+constexpr std::size_t kSampleBytes = kSampleMax * (4U + 4U + 4U * sizeof(double));
+static __device__ __align__(256) unsigned char scratch[kSampleBytes];
+
+constexpr int kMaxDevices = 64;
+std::once_flag scratch_once[kMaxDevices];
+void* scratch_ptr[kMaxDevices] = {};
+hipError_t scratch_status[kMaxDevices] = {};
+std::mutex scratch_lock[kMaxDevices];  // one call at a time per device shares its array
+
+// Boundary slot of each listed node (bcell is sorted: the boundary list is in cell order), or
+// -1: the host loop of lbm3d_sample, as a binary search on the device.
+__global__ __launch_bounds__(256) void sample_slots(const std::uint32_t* __restrict__ bcell, std::uint32_t nb,
+                                                    const std::uint32_t* __restrict__ cells, std::uint32_t n,
+                                                    std::int32_t* __restrict__ slots) {
+    const std::uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) {
+        return;
+    }
+    const std::uint32_t cell = cells[k];
+    std::uint32_t lo = 0;
+    std::uint32_t hi = nb;
+    while (lo < hi) {
+        const std::uint32_t mid = lo + ((hi - lo) >> 1);
+        if (bcell[mid] < cell) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    slots[k] = lo < nb && bcell[lo] == cell ? static_cast<std::int32_t>(lo) : -1;
+}
+
+} // namespace
+
+extern "C" {
+
+int lbm3d_sample_fixed(lbm3d* h, const uint32_t* cells, size_t n, double* rho, double* ux, double* uy, double* uz) {
+    if (h == nullptr || (n > 0 && cells == nullptr)) {
+        return fail(LBM_EINVAL, "lbm3d: null argument");
+    }
+    if (n == 0) {
+        return ok();
+    }
+    if (n > kSampleMax) {
+        return fail(LBM_EINVAL, "lbm3d: lbm3d_sample_fixed takes at most " + std::to_string(kSampleMax) +
+                                    " nodes per call (" + std::to_string(n) + " given); use lbm3d_sample");
+    }
+    for (std::size_t k = 0; k < n; ++k) {
+        if (cells[k] >= h->cells) {
+            return fail(LBM_EINVAL, "lbm3d: sample node outside the lattice");
+        }
+    }
+    if (h->device < 0 || h->device >= kMaxDevices) {
+        return fail(LBM_EINVAL, "lbm3d: lbm3d_sample_fixed supports device ordinals below " +
+                                    std::to_string(kMaxDevices));
+    }
+    const int dev = h->device;
+    std::call_once(scratch_once[dev], [dev] {
+        int current = 0;
+        hipError_t e = hipGetDevice(&current);
+        if (e == hipSuccess && current != dev) {
+            e = hipSetDevice(dev);
+        }
+        if (e == hipSuccess) {
+            e = hipGetSymbolAddress(&scratch_ptr[dev], HIP_SYMBOL(scratch));
+        }
+        if (current != dev) {
+            (void)hipSetDevice(current);
+        }
+        scratch_status[dev] = e;
+    });
+    if (scratch_status[dev] != hipSuccess || scratch_ptr[dev] == nullptr) {
+        return hip_fail("sample scratch address", scratch_status[dev] != hipSuccess ? scratch_status[dev]
+                                                                                    : hipErrorInvalidSymbol);
+    }
+    const std::lock_guard<std::mutex> guard(scratch_lock[dev]);
+    char* base = static_cast<char*>(scratch_ptr[dev]);
+    auto* dcells = reinterpret_cast<std::uint32_t*>(base);
+    auto* dslots = reinterpret_cast<std::int32_t*>(base + 4U * kSampleMax);
+    auto* dout = reinterpret_cast<double*>(base + 8U * kSampleMax);
+    const auto count = static_cast<std::uint32_t>(n);
+    hipError_t e = hipMemcpyAsync(dcells, cells, 4U * n, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(sample_slots, dim3(static_cast<std::uint32_t>((n + 255U) / 256U)), dim3(256), 0, h->stream,
+                           h->bcell, static_cast<std::uint32_t>(h->tables.bcell.size()), dcells, count, dslots);
+        e = hipGetLastError();
+    }
+    int rc = e == hipSuccess ? LBM_OK : hip_fail("sample upload", e);
+    if (rc == LBM_OK) {
+        const int phase = static_cast<int>(h->steps & 1U);
+        const double force[3] = {h->cfg.body_force_x, h->cfg.body_force_y, h->cfg.body_force_z};
+        rc = with_store(*h, [&]<class T>() -> int {
+            const hipError_t err = lbm::d3q19_sample<T>(descriptor<T>(*h), phase, force, dcells, dslots, count, dout,
+                                                        h->stream);
+            return err == hipSuccess ? ok() : hip_fail("sample", err);
+        });
+    }
+    double* outs[4] = {rho, ux, uy, uz};
+    for (int k = 0; k < 4 && rc == LBM_OK; ++k) {
+        if (outs[k] != nullptr) {
+            e = hipMemcpyAsync(outs[k], dout + k * n, n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+            if (e != hipSuccess) {
+                rc = hip_fail("sample readback", e);
+            }
+        }
+    }
+    const int sy = sync(*h, "sample readback");
     return rc != LBM_OK ? rc : sy;
 }
 
