@@ -180,9 +180,9 @@ __device__ __forceinline__ void bulk_cell2(const Lattice2<typename T::R>& p, std
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         if constexpr (kLocal) {
-            f[L2::opp(i) * stride + cell] = T::pack(v[i]);
+            f[L2::opp(i) * stride + cell] = T::nearest(v[i]);
         } else {
-            f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off2(i, nx))] = T::pack(v[i]);
+            f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off2(i, nx))] = T::nearest(v[i]);
         }
     }
 }
@@ -251,7 +251,7 @@ __device__ __forceinline__ void boundary_node2(const Lattice2<typename T::R>& p,
     collide2<T>(v, p);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        const S packed = T::pack(v[i]);
+        const S packed = T::nearest(v[i]);
         sw[i * bs + b] = T::load(packed);
         if constexpr (kLocal) {
             f[L2::opp(i) * stride + cell] = packed;

@@ -107,20 +107,30 @@ std::uint32_t bulk_lds(const lbm3d& h, bool local) {
 
 // One step under the handle's schedule: one fused launch; the bulk and boundary kernels one
 // after the other; or the two kernels concurrently on two streams.
+// The end of a step: dithered formats advance their step clock (after both halves of the
+// step, on the handle's stream).
+template <class T> hipError_t end_step(lbm3d& h) {
+    if constexpr (T::dithered) {
+        return lbm::d3q19_tick(h.clock, h.stream);
+    }
+    return hipSuccess;
+}
+
 template <class T> hipError_t step_once(lbm3d& h, const lbm::Lattice3<typename T::R>& p, bool local) {
+    hipError_t e = hipSuccess;
     if (h.schedule == LBM_SCHEDULE_FUSED) {
-        return lbm::d3q19_step_fused<T>(p, local, h.stream);
+        e = lbm::d3q19_step_fused<T>(p, local, h.stream);
+    } else if (h.schedule == LBM_SCHEDULE_SERIAL || p.nb == 0) {
+        e = lbm::d3q19_step<T>(p, local, h.block, local ? h.cpt : h.cpt_b, bulk_lds(h, local), h.stream);
+    } else {
+        e = hipEventRecord(h.fork, h.stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h.side, h.fork, 0);
+        if (e == hipSuccess) e = lbm::d3q19_step_boundary<T>(p, local, h.side);
+        if (e == hipSuccess) e = lbm::d3q19_step_bulk<T>(p, local, h.block, local ? h.cpt : h.cpt_b, bulk_lds(h, local), h.stream);
+        if (e == hipSuccess) e = hipEventRecord(h.join, h.side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h.stream, h.join, 0);
     }
-    if (h.schedule == LBM_SCHEDULE_SERIAL || p.nb == 0) {
-        return lbm::d3q19_step<T>(p, local, h.block, local ? h.cpt : h.cpt_b, bulk_lds(h, local), h.stream);
-    }
-    hipError_t e = hipEventRecord(h.fork, h.stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(h.side, h.fork, 0);
-    if (e == hipSuccess) e = lbm::d3q19_step_boundary<T>(p, local, h.side);
-    if (e == hipSuccess) e = lbm::d3q19_step_bulk<T>(p, local, h.block, local ? h.cpt : h.cpt_b, bulk_lds(h, local), h.stream);
-    if (e == hipSuccess) e = hipEventRecord(h.join, h.side);
-    if (e == hipSuccess) e = hipStreamWaitEvent(h.stream, h.join, 0);
-    return e;
+    return e == hipSuccess ? end_step<T>(h) : e;
 }
 
 int sync(lbm3d& h, const char* what) {
@@ -148,7 +158,10 @@ int initialize_mode(lbm3d* h, int mode, const double* host_planes, std::uint64_t
         }
     }
     const int rc = with_store(*h, [&]<class T>() -> int {
-        const hipError_t e = lbm::d3q19_initialize<T>(descriptor<T>(*h), mode, planes, u0, seed, amplitude, h->stream);
+        hipError_t e = lbm::d3q19_initialize<T>(descriptor<T>(*h), mode, planes, u0, seed, amplitude, h->stream);
+        if (e == hipSuccess) {
+            e = hipMemsetAsync(h->clock, 0, sizeof(std::uint32_t), h->stream);
+        }
         return e == hipSuccess ? ok() : hip_fail("initialize", e);
     });
     const int done = sync(*h, "initialize");
@@ -294,6 +307,7 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
         return rc;
     }
     h->result = h->partial + red;
+    h->clock = reinterpret_cast<std::uint32_t*>(h->result + 7); // results use at most 6 doubles
     // Upload the tables.
     std::vector<std::uint32_t> lcell(nl);
     std::vector<std::uint8_t> ldir(nl);
@@ -897,6 +911,9 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
                 if (err == hipSuccess && per_kernel) {
                     err = hipEventRecord(ev[3U * n + 3U], h->stream);
                 }
+                if (err == hipSuccess) {
+                    err = end_step<T>(*h);
+                }
                 return err == hipSuccess ? ok() : hip_fail("timed step", err);
             });
             ++h->steps;
@@ -966,6 +983,7 @@ int lbm3d_time_kernels(lbm3d* h, uint64_t count, double ms[4]) {
             if (err == hipSuccess) err = hipEventRecord(ev[3U * n + 1U], h->stream);
             if (err == hipSuccess) err = lbm::d3q19_step_boundary<T>(p, local, h->stream);
             if (err == hipSuccess) err = hipEventRecord(ev[3U * n + 2U], h->stream);
+            if (err == hipSuccess) err = end_step<T>(*h);
             return err == hipSuccess ? ok() : hip_fail("timed kernels", err);
         });
         if (rc == LBM_OK) {
@@ -1065,7 +1083,7 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
     const std::size_t hist_bytes = 5U * static_cast<std::size_t>(h->tables.bstride) * rb;
     char* f_base = static_cast<char*>(h->f) - guard * sb;
     void* snap = nullptr;
-    if (hipMalloc(&snap, f_bytes + stash_bytes + hist_bytes) != hipSuccess) {
+    if (hipMalloc(&snap, f_bytes + stash_bytes + hist_bytes + sizeof(std::uint32_t)) != hipSuccess) {
         (void)hipGetLastError();
         return fail(LBM_ENOMEM, "lbm3d: autotune needs a device copy of the lattice state (" +
                                     lbm::host::megabytes(f_bytes + stash_bytes + hist_bytes) + ")");
@@ -1074,6 +1092,10 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
     hipError_t e = hipMemcpyAsync(s, f_base, f_bytes, hipMemcpyDeviceToDevice, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s + f_bytes, h->stash, stash_bytes, hipMemcpyDeviceToDevice, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s + f_bytes + stash_bytes, h->hist, hist_bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) {
+        e = hipMemcpyAsync(s + f_bytes + stash_bytes + hist_bytes, h->clock, sizeof(std::uint32_t),
+                           hipMemcpyDeviceToDevice, h->stream);
+    }
     if (e != hipSuccess) {
         (void)hipFree(snap);
         return hip_fail("autotune snapshot", e);
@@ -1190,6 +1212,10 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
     e = hipMemcpyAsync(f_base, s, f_bytes, hipMemcpyDeviceToDevice, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(h->stash, s + f_bytes, stash_bytes, hipMemcpyDeviceToDevice, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(h->hist, s + f_bytes + stash_bytes, hist_bytes, hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) {
+        e = hipMemcpyAsync(h->clock, s + f_bytes + stash_bytes + hist_bytes, sizeof(std::uint32_t),
+                           hipMemcpyDeviceToDevice, h->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(snap);
     h->steps = saved_steps;

@@ -60,6 +60,12 @@ __global__ __launch_bounds__(256) void d3q19_boundary(Lattice3<typename T::R> p)
     }
 }
 
+__global__ void d3q19_tick_kernel(std::uint32_t* clock) {
+    if (threadIdx.x == 0) {
+        *clock += 1U;
+    }
+}
+
 // ---- state initialisation ------------------------------------------------------------------
 // Equilibrium of (rho, u) per node in the post-A layout; solids at rest. mode 0: uniform
 // state (density 1, velocity u0); 1: moment planes in `planes`; 2: u0 plus the seeded
@@ -577,6 +583,11 @@ hipError_t d3q19_step_boundary(const Lattice3<typename T::R>& p, bool local, hip
     } else {
         hipLaunchKernelGGL((d3q19_boundary<T, false>), dim3(grid), dim3(256), 0, s, p);
     }
+    return hipGetLastError();
+}
+
+hipError_t d3q19_tick(std::uint32_t* clock, hipStream_t s) {
+    hipLaunchKernelGGL(d3q19_tick_kernel, dim3(1), dim3(64), 0, s, clock);
     return hipGetLastError();
 }
 

@@ -336,12 +336,14 @@ __device__ __forceinline__ void bulk_finish(const Lattice3<typename T::R>& p, st
         sponge = sponge_sigma3(p, rest - y * p.nx, y, z);
     }
     collide3<T>(v, p, sponge);
+    const std::uint32_t bits = T::dithered ? cell_dither(cell, slot_dither(*p.clock)) : 0U;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         if constexpr (kLocal) {
-            f[L3::opp(i) * stride + cell] = T::pack(v[i]);
+            f[L3::opp(i) * stride + cell] = T::pack(v[i], direction_dither(bits, i));
         } else {
-            f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] = T::pack(v[i]);
+            f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] =
+                T::pack(v[i], direction_dither(bits, i));
         }
     }
 }
@@ -363,11 +365,14 @@ template <class T, int V> __device__ __forceinline__ void load_run(const typenam
     }
 }
 
-template <class T, int V> __device__ __forceinline__ void store_run(typename T::S* dst, const typename T::R (&in)[V]) {
+// Direction i of V consecutive cells, whose dither bits are `bits`.
+template <class T, int V>
+__device__ __forceinline__ void store_run(typename T::S* dst, const typename T::R (&in)[V], const std::uint32_t (&bits)[V],
+                                          int i) {
     typename T::S tmp[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) {
-        tmp[k] = T::pack(in[k]);
+        tmp[k] = T::pack(in[k], direction_dither(bits[k], i));
     }
     __builtin_memcpy(dst, tmp, sizeof(tmp));
 }
@@ -442,6 +447,14 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
             collide3<T>(v[c], p, sponge);
         }
     }
+    std::uint32_t bits[V];
+    {
+        const std::uint32_t epoch = T::dithered ? slot_dither(*p.clock) : 0U;
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            bits[c] = T::dithered ? cell_dither(c0 + static_cast<std::uint32_t>(c), epoch) : 0U;
+        }
+    }
     if (all) {
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
@@ -451,11 +464,11 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
                 run[c] = v[c][i];
             }
             if constexpr (kLocal) {
-                store_run<T, V>(element(f + static_cast<std::size_t>(L3::opp(i)) * stride, c0), run);
+                store_run<T, V>(element(f + static_cast<std::size_t>(L3::opp(i)) * stride, c0), run, bits, i);
             } else {
                 store_run<T, V>(element(f + static_cast<std::size_t>(i) * stride,
                                         static_cast<std::uint32_t>(static_cast<int>(c0) + off3(i, nx, plane))),
-                                run);
+                                run, bits, i);
             }
         }
     } else {
@@ -466,9 +479,10 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
                     if constexpr (kLocal) {
-                        f[L3::opp(i) * stride + cell] = T::pack(v[c][i]);
+                        f[L3::opp(i) * stride + cell] = T::pack(v[c][i], direction_dither(bits[c], i));
                     } else {
-                        f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] = T::pack(v[c][i]);
+                        f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] =
+                            T::pack(v[c][i], direction_dither(bits[c], i));
                     }
                 }
             }
@@ -607,10 +621,11 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
     }
     const R sponge = p.sponge_strength > R(0) ? sponge_sigma3(p, x, y, z) : R(0);
     collide3<T>(v, p, sponge);
+    const std::uint32_t bits = T::dithered ? cell_dither(cell, slot_dither(*p.clock)) : 0U;
     const std::uint32_t wmask = p.bwrite[b];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        const S packed = T::pack(v[i]);
+        const S packed = T::pack(v[i], direction_dither(bits, i));
         sw[i * bs + b] = T::load(packed);
         if (((wmask >> i) & 1U) == 0U) {
             continue;

@@ -87,9 +87,11 @@ __global__ __launch_bounds__(256) void batch_boundary(const Lattice3<typename T:
     }
 }
 
+// parity[l] holds lattice l's step count (mod 2^32): bit 0 its A-A parity, the whole word the
+// step clock of the 16-bit formats' rounding dither.
 __global__ void batch_advance(std::uint32_t* parity, std::uint32_t n) {
     for (std::uint32_t l = threadIdx.x; l < n; l += blockDim.x) {
-        parity[l] ^= 1U;
+        parity[l] += 1U;
     }
 }
 
@@ -321,18 +323,22 @@ template <class T> int fill_group(lbm_batch& b, Group& g) {
         bulk[k + 1] = bulk[k] + lbm::ceil_div(lat[k].cells, per);
         bnd[k + 1] = bnd[k] + std::max<std::uint32_t>(1, lbm::ceil_div(lat[k].nb, 256));
         lnk[k + 1] = lnk[k] + std::max<std::uint32_t>(1, lbm::ceil_div(v.count, 256));
-        parity[k] = static_cast<std::uint32_t>(lbm3d_steps(h) & 1U);
+        parity[k] = static_cast<std::uint32_t>(lbm3d_steps(h)); // the step clock too (mod 2^32)
     }
     g.bulk_blocks = bulk[n];
     g.bnd_blocks = bnd[n];
     g.link_blocks = lnk[n];
     int rc = LBM_OK;
+    // (the descriptors' clocks point into g.parity once it is allocated, below)
     if ((rc = dalloc(b, reinterpret_cast<char**>(&g.lat), n * sizeof(lbm::Lattice3<R>))) ||
         (rc = dalloc(b, &g.links, n * sizeof(lbm::LinkTable))) || (rc = dalloc(b, &g.bulk_start, (n + 1) * 4)) ||
         (rc = dalloc(b, &g.bnd_start, (n + 1) * 4)) || (rc = dalloc(b, &g.link_start, (n + 1) * 4)) ||
         (rc = dalloc(b, &g.parity, n * 4)) || (rc = dalloc(b, &g.partial, static_cast<std::size_t>(g.link_blocks) * 48)) ||
         (rc = dalloc(b, &g.force, n * 48))) {
         return rc;
+    }
+    for (std::size_t k = 0; k < n; ++k) {
+        lat[k].clock = g.parity + k;
     }
     const auto up = [&](void* dst, const void* src, std::size_t bytes) {
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b.stream);

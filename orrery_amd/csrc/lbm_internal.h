@@ -61,6 +61,9 @@ struct lbm3d {
     std::uint32_t* tile_slot = nullptr;
     double* partial = nullptr;
     double* result = nullptr;
+    // Steps since initialisation, on the device (after the reduction results): the 16-bit
+    // formats draw their rounding dither from it; a one-thread kernel advances it each step.
+    std::uint32_t* clock = nullptr;
     std::size_t device_bytes = 0;
     std::vector<void*> owned;
     // Replayed runs of steps of lbm3d_step (serial and fused schedules).
@@ -157,6 +160,7 @@ template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
     p.stash = static_cast<R*>(h.stash);
     p.hist = static_cast<R*>(h.hist);
     p.tile_slot = h.tile_slot;
+    p.clock = h.clock;
     p.nx = static_cast<std::uint32_t>(c.nx);
     p.ny = static_cast<std::uint32_t>(c.ny);
     p.nz = static_cast<std::uint32_t>(c.nz);

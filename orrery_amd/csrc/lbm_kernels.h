@@ -21,6 +21,8 @@ hipError_t d3q19_step_bulk(const Lattice3<typename T::R>& p, bool local, int blo
                            hipStream_t s);
 template <class T>
 hipError_t d3q19_step_boundary(const Lattice3<typename T::R>& p, bool local, hipStream_t s);
+// Advances a lattice's step clock by one (the end of a step of a dithered 16-bit format).
+hipError_t d3q19_tick(std::uint32_t* clock, hipStream_t s);
 // The whole step in one launch (bulk and boundary nodes in cell order).
 template <class T>
 hipError_t d3q19_step_fused(const Lattice3<typename T::R>& p, bool local, hipStream_t s);
