@@ -14,9 +14,19 @@ same lattice on all host cores. Each storage format (f32, f16s, bf16s) is also r
 2000 and its drag force and velocity field compared with the f64 storage's (the FP64 oracle
 arithmetic on the device) at that step.
 
-Other workloads (--workload): polar (the batched wing-section sweep), serve (the
-continuous-batching server), d2q9 (the 400 x 160 cylinder channel), sphere (the 256 x 128 x 128
-sphere channel).
+The same line carries the metric's second half, measured in the same run:
+- `batched`: the 48 NACA 0012 wing sections (96^3 .. 320 x 160 x 160, FP32) advanced in one
+  batch to convergence, against the same solves one lattice at a time (every 8th timed, scaled
+  by cell-steps): converged solves per hour both ways, the speed-up, and the Cd of the 8
+  smallest converged sections against the FP64 device lattice over the same samples;
+- `served`: 16 requests with seeded Poisson arrivals at 4/s (sphere, cone, cylinder, capsule,
+  wedge; 64^3 .. 384 x 192 x 192; f32 or f16s) through the continuous-batching server:
+  requests/s, converged solves per hour, p50/p99 latency from arrival, the one-at-a-time
+  estimate, and the Cd of the 8 smallest converged requests against the FP64 device lattice.
+
+Other workloads (--workload): polar (the batched section alone), serve (the served section
+alone), d2q9 (the 400 x 160 cylinder channel), sphere (the 256 x 128 x 128 sphere channel),
+reference (the reference's own GPU benchmarks).
 """
 from __future__ import annotations
 
@@ -143,8 +153,21 @@ def cpu_baseline(config, mask, steps, seconds=0.0):
     del oracle
     cells = config.cells
     return {"value": cells * steps / dt / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{steps} FP64 steps of the same {config.nx}x{config.ny}x{config.nz} lattice after 1 warm-up "
-                      f"step ({dt:.1f} s), OpenMP over rows, host CPU {platform.processor() or platform.machine()}"}
+                      f"step ({dt:.1f} s), OpenMP over rows, host CPU {cpu_model()}"}
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for row in f:
+                if row.startswith("model name"):
+                    return row.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def copy_calibration(mib):
@@ -177,49 +200,131 @@ def log(message):
     print(f"bench: {message}", file=sys.stderr, flush=True)
 
 
-def polar_bench(args):
-    """Batched Cd solves per hour: the wing-section sweep in one batch against the same cases one
-    lattice at a time (host-paced force reads), same kernels, same convergence test."""
+def batched_section(n_lattices, sequential_every, fp64_checks, storage="f32"):
+    """The batched wing-section sweep (48 NACA 0012 sections at 96^3 .. 320x160x160) run to
+    convergence in one batch, against the same solves one lattice at a time (host-paced force
+    reads, same kernels and convergence test), timed on every `sequential_every`-th case and
+    scaled by work (cell-steps); the Cd of the `fp64_checks` smallest converged sections against
+    the FP64 device lattice over the same samples."""
     from orrery_amd import polar
-    cases = polar.wing_cases(args.lattices)
-    log(f"polar: voxelising {len(cases)} wing sections")
-    lattices = [polar.wing_lattice(c) for c in cases]
-    log("polar: batched run")
-    batched, tb = polar.run_batched(cases, lattices, log=log)
-    pick = list(range(0, len(cases), max(1, args.sequential_every)))
-    log(f"polar: one-at-a-time run of {len(pick)} cases")
-    sequential, ts = polar.run_sequential([cases[k] for k in pick], [lattices[k] for k in pick], log=log)
-    # The one-at-a-time rate on the cases it ran, scaled by their share of the batched work
-    # (cell-steps), so a subset of cases gives a like-for-like estimate.
+    cases = polar.wing_cases(n_lattices)
+    log(f"batched: voxelising {len(cases)} wing sections on the device")
+    lattices = [polar.wing_lattice(c, device=True) for c in cases]
+    batched, tb = polar.run_batched(cases, lattices, storage, log=log)
+    pick = list(range(0, len(cases), max(1, sequential_every)))
+    log(f"batched: one-at-a-time run of {len(pick)} cases")
+    sequential, ts = polar.run_sequential([cases[k] for k in pick], [lattices[k] for k in pick], storage, log=log)
     work = {s.case.index: s.case.cells * s.steps for s in batched}
     share = sum(work[k] for k in pick) / sum(work.values())
     seq_total = ts / share
+    seq_glups = sum(work[k] for k in pick) / ts / 1e9
     by_case = {s.case.index: s for s in batched}
     same = all((s.steps, s.cd, s.cl) == (by_case[s.case.index].steps, by_case[s.case.index].cd,
                                          by_case[s.case.index].cl) for s in sequential)
-    rate = len(batched) / tb * 3600.0
-    from orrery_amd import experiment
-    comparison_table = experiment.compare_by_geometry(polar.records(batched))[0]
-    if args.csv:
-        with open(args.csv, "w", newline="") as f:
-            f.write(polar.comparison_csv(batched))
-        log(f"polar: comparison table written to {args.csv}")
-    line = {
-        "metric": "batched Cd solves per hour (wing-section sweep, convergence |dCd| < 1e-4 over 500 steps)",
-        "value": round(rate, 1), "unit": "solves/h", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{len(cases)} NACA 0012 sections, dims in [96^3, 320x160x160], chord 48-96 cells, "
-                               "AoA -4..18 deg, Re 500-2000, LES, interpolated wall, periodic span",
-                   "cells": sum(c.cells for c in cases), "parallelism": "single GPU"},
-        "batched_s": round(tb, 2), "sequential_s_estimated": round(seq_total, 2),
-        "sequential_cases_timed": len(pick), "speedup": round(seq_total / tb, 3),
-        "same_results_as_sequential": same,
-        "converged": sum(s.converged for s in batched),
+    done = [s for s in batched if s.converged]
+    errors = []
+    for s in sorted(done, key=lambda s: s.case.cells)[:fp64_checks]:
+        cd64, cl64 = polar.fp64_coefficients(s.case, lattices[s.case.index], s.steps, s.mean_from)
+        errors.append({"case": s.case.index, "cells": s.case.cells, "steps": s.steps, "kind": s.kind,
+                       "cd": round(s.cd, 6), "cd_fp64": round(cd64, 6), "cd_rel_error": float(f"{abs(s.cd / cd64 - 1):.3e}")})
+        log(f"batched: case {s.case.index} ({s.kind}) Cd {s.cd:.6f} vs FP64 {cd64:.6f}")
+    kinds = {}
+    for s in done:
+        kinds[s.kind] = kinds.get(s.kind, 0) + 1
+    plain_fixed = sum(c.cells * polar.MAX_STEPS for c in cases) / (seq_glups * 1e9)
+    return {
+        "workload": f"{len(cases)} NACA 0012 sections, dims in [96^3, 320x160x160], chord 48-96 cells, AoA -4..18 deg, "
+                    f"Re 500-2000, LES, interpolated wall, periodic span, {storage}",
+        "cells": sum(c.cells for c in cases), "converged": len(done), "converged_kinds": kinds,
+        "batched_s": round(tb, 2), "converged_solves_per_hour": round(len(done) / tb * 3600.0, 1),
+        "sequential_s_estimated": round(seq_total, 2), "sequential_cases_timed": len(pick),
+        "sequential_converged_solves_per_hour": round(len(done) / seq_total * 3600.0, 1),
+        "speedup": round(seq_total / tb, 3), "same_results_as_sequential": same,
+        "glups_batched": round(sum(work.values()) / tb / 1e9, 2), "glups_sequential": round(seq_glups, 2),
+        "sequential_fixed_duration_s_estimated": round(plain_fixed, 1),
+        "speedup_vs_fixed_duration": round(plain_fixed / tb, 3),
         "lift_slope_per_rad": round(polar.lift_slope(batched), 4),
-        "lift_slope_all_angles_per_rad": round(comparison_table.lift_slope_per_rad, 4)
-        if comparison_table.lift_slope_per_rad is not None else None,
-        "glups_batched": round(sum(work.values()) / tb / 1e9, 2),
+        "cd_vs_fp64": errors,
+        "cd_rel_error_vs_fp64_max": max(e["cd_rel_error"] for e in errors) if errors else None,
     }
+
+
+def served_section(n_requests, rate, fp64_checks, sequential_every):
+    """The continuous-batching server on `n_requests` seeded Poisson arrivals: throughput,
+    latency percentiles, converged solves per hour, the same requests one at a time (every
+    `sequential_every`-th, timed alone to the same step count, scaled by work), and the Cd of
+    the `fp64_checks` smallest converged requests against the FP64 device lattice over the same
+    samples."""
+    import numpy as np
+    from orrery_amd import Lbm3DSolver, serve
+    reqs = serve.requests(n_requests, rate)
+    server = serve.Server()
+    served, wall = server.run(reqs, log=log)
+    lat = np.array([s.latency for s in served])
+    queued = np.array([s.queued for s in served])
+    first = min(r.arrival for r in reqs)
+    last = max(s.finished for s in served)
+    done = [s for s in served if s.converged]
+    # one request at a time: its lattice alone, host-paced samples, to the step it was served at
+    pick = sorted(served, key=lambda s: s.request.id)[::max(1, sequential_every)]
+    log(f"served: one-at-a-time run of {len(pick)} requests")
+    seq_stepping = 0.0  # stepping and host-paced force reads only, not the lattice builds
+    for s in pick:
+        config, mask = serve.request_lattice(s.request)
+        solver = Lbm3DSolver(config, mask, s.request.storage)
+        solver.initialize_perturbed(server.seed, 1e-4)
+        t1 = time.perf_counter()
+        while solver.steps() < s.steps:
+            solver.step(serve.SAMPLE_EVERY)
+            solver.force()
+        seq_stepping += time.perf_counter() - t1
+        solver.close()
+    work_all = sum(s.request.cells * s.steps for s in served)
+    work_pick = sum(s.request.cells * s.steps for s in pick)
+    seq_total = seq_stepping * work_all / work_pick
+    span = last - first
+    errors = []
+    for s in sorted(done, key=lambda s: s.request.cells)[:fp64_checks]:
+        ref = serve.fp64_reference_cd(s.request, s.steps, mean_from=s.mean_from)
+        errors.append({"request": s.request.id, "cells": s.request.cells, "storage": s.request.storage,
+                       "steps": s.steps, "kind": s.kind, "cd": round(s.cd, 6), "cd_fp64": round(ref, 6),
+                       "cd_rel_error": float(f"{abs(s.cd / ref - 1):.3e}")})
+        log(f"served: request {s.request.id} {s.request.storage} ({s.kind}) Cd {s.cd:.6f} vs FP64 {ref:.6f}")
+    return {
+        "workload": f"{n_requests} requests, Poisson lambda {rate}/s, bodies {'/'.join(serve.BODIES)}, dims in "
+                    "[64^3, 384x192x192], AoA 0-30 deg, Re 200-3000, f32 or f16s per request, HBM budget 240 GB",
+        "served": len(served), "converged": len(done), "wall_s": round(wall, 2),
+        "requests_per_s": round(len(served) / span, 4),
+        "converged_solves_per_hour": round(len(done) / span * 3600.0, 1),
+        "latency_p50_s": round(float(np.percentile(lat, 50)), 3), "latency_p99_s": round(float(np.percentile(lat, 99)), 3),
+        "queued_p50_s": round(float(np.percentile(queued, 50)), 3), "queued_p99_s": round(float(np.percentile(queued, 99)), 3),
+        "gpu_busy_fraction": round(server.busy / wall, 3), "offered_per_s": rate,
+        "sequential_requests_timed": len(pick), "work_share_timed": round(work_pick / work_all, 4),
+        "sequential_s_estimated": round(seq_total, 2),
+        "sequential_converged_solves_per_hour": round(len(done) / seq_total * 3600.0, 1),
+        "speedup": round(seq_total / span, 3),
+        "glups_served": round(work_all / span / 1e9, 2), "glups_sequential": round(work_pick / seq_stepping / 1e9, 2),
+        "cd_vs_fp64": errors,
+        "cd_rel_error_vs_fp64_max": max(e["cd_rel_error"] for e in errors) if errors else None,
+    }
+
+
+def polar_bench(args):
+    """Batched Cd solves per hour of the wing-section sweep alone (the `batched` section of the
+    default line), with the comparison table as CSV on request."""
+    from orrery_amd import experiment, polar
+    section = batched_section(args.lattices, args.sequential_every, args.fp64_checks)
+    line = {"metric": "batched Cd solves per hour (wing-section sweep)", "value": section["converged_solves_per_hour"],
+            "unit": "solves/h", "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic", "config": {"workload": section["workload"], "parallelism": "single GPU"},
+            "batched": section}
+    if args.csv:
+        cases = polar.wing_cases(args.lattices)
+        log("polar: the comparison table needs the solves; rerunning the batch for the CSV")
+        solves, _ = polar.run_batched(cases, [polar.wing_lattice(c, device=True) for c in cases], log=log)
+        with open(args.csv, "w", newline="") as f:
+            f.write(polar.comparison_csv(solves))
+        line["lift_slope_all_angles_per_rad"] = experiment.compare_by_geometry(polar.records(solves))[0].lift_slope_per_rad
     print(json.dumps(line), flush=True)
     return 0
 
@@ -435,12 +540,13 @@ def sphere_bench(args):
 
 
 def serve_bench(args):
-    from orrery_amd import serve
-    if args.admission == "all":
-        result = serve.serve_bench(args.requests, log=log, order="arrival", max_active_cells=None)
-    else:
-        result = serve.serve_bench(args.requests, log=log)
-    print(json.dumps(result), flush=True)
+    """The continuous-batching server alone (the `served` section of the default line)."""
+    section = served_section(args.requests, 4.0, args.fp64_checks, args.served_sequential_every)
+    line = {"metric": "served Cd requests per second (continuous batching, Poisson arrivals)",
+            "value": section["requests_per_s"], "unit": "requests/s", "n_gpus": 1, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32/f16s", "data": "synthetic",
+            "config": {"workload": section["workload"], "parallelism": "single GPU"}, "served": section}
+    print(json.dumps(line), flush=True)
     return 0
 
 
@@ -587,13 +693,16 @@ def main(argv=None):
                          "reference's own GPU benchmarks and adapter solves beside its published numbers")
     ap.add_argument("--d2q9-steps", type=int, default=20000, help="d2q9: configured steps (the run is twice as long)")
     ap.add_argument("--sphere-steps", type=int, default=5000, help="sphere: steps of the sampled runs")
-    ap.add_argument("--lattices", type=int, default=48, help="polar: number of wing cases")
-    ap.add_argument("--sequential-every", type=int, default=1,
-                    help="polar: time the one-at-a-time baseline on every k-th case (k > 1 bounds the run)")
-    ap.add_argument("--requests", type=int, default=32, help="serve: number of requests")
-    ap.add_argument("--admission", choices=["shortest", "all"], default="shortest",
-                    help="serve: shortest work first under a cap on resident nodes, or every request that fits, "
-                         "oldest first")
+    ap.add_argument("--lattices", type=int, default=48,
+                    help="wing sections of the batched sweep (the `batched` section; 0 leaves it out)")
+    ap.add_argument("--sequential-every", type=int, default=8,
+                    help="time the one-at-a-time baseline of the sweep on every k-th section")
+    ap.add_argument("--requests", type=int, default=16,
+                    help="requests of the served workload (the `served` section; 0 leaves it out)")
+    ap.add_argument("--served-sequential-every", type=int, default=4,
+                    help="time every k-th served request alone for the one-at-a-time baseline")
+    ap.add_argument("--fp64-checks", type=int, default=8,
+                    help="converged solves (the smallest) whose Cd is checked against the FP64 device lattice")
     ap.add_argument("--csv", default="", help="polar: write the comparison table (CSV) of the batched solves here")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device with a plain copy kernel (3 launches), the "
@@ -651,7 +760,8 @@ def main(argv=None):
         cpu = cpu_baseline(config, mask, args.cpu_steps, args.cpu_seconds)
     traffic = measured_traffic()
     line = {
-        "metric": "D3Q19 GLUPS (FP32 A-A, BGK+Smagorinsky) and % HBM roofline",
+        "metric": "D3Q19 GLUPS (FP32 A-A, BGK+Smagorinsky) and % HBM roofline; batched Cd solves/hour, p50/p99 "
+                  "latency @1% Cd error",
         "value": round(glups, 3),
         "unit": "GLUPS",
         "n_gpus": 1,
@@ -690,6 +800,8 @@ def main(argv=None):
             "step_roofline_frac": round(bytes_per_node * cells / (main_r["ms_per_step"] * 1e-3) / HBM_PEAK, 4),
         },
         "cpu_baseline": cpu,
+        "batched": None,
+        "served": None,
         "storage_variants": {
             k: {"glups": round(cells / (r["ms_per_step"] * 1e-3) / 1e9, 3),
                 "bytes_per_node": 2 * 19 * {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[k],
@@ -701,6 +813,17 @@ def main(argv=None):
             for k, r in results.items()
         },
     }
+    # The metric's second half: batched and served converged Cd solves, measured in this run.
+    if args.lattices > 0:
+        t0 = time.perf_counter()
+        log(f"batched: the {args.lattices}-section wing sweep")
+        line["batched"] = batched_section(args.lattices, args.sequential_every, args.fp64_checks)
+        line["batched"]["section_s"] = round(time.perf_counter() - t0, 1)
+    if args.requests > 0:
+        t0 = time.perf_counter()
+        log(f"served: {args.requests} requests at 4/s")
+        line["served"] = served_section(args.requests, 4.0, args.fp64_checks, args.served_sequential_every)
+        line["served"]["section_s"] = round(time.perf_counter() - t0, 1)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -139,7 +139,8 @@ class Solve:
     cl: float
     converged: bool
     finished_at: float  # seconds from the start of the run
-    kind: str = ""      # "steady" or "settled" when converged
+    kind: str = ""      # "steady", "settled" or "stationary" when converged
+    mean_from: float = 0.0  # first step of the samples the coefficients average
 
 
 def run_batched(cases: list[WingCase], lattices, storage: str = "f32", budget_bytes: int = 240 * 10 ** 9,
@@ -167,7 +168,8 @@ def run_batched(cases: list[WingCase], lattices, storage: str = "f32", budget_by
             v = judge(case, t, f)
             if v.converged or t[-1] >= MAX_STEPS:
                 cd, cl = (v.drag, v.lift) if v.converged else [float(c[-WINDOW:].mean()) for c in coefficients(case, f)]
-                done.append(Solve(case, int(t[-1]), cd, cl, v.converged, time.perf_counter() - t0, v.kind))
+                done.append(Solve(case, int(t[-1]), cd, cl, v.converged, time.perf_counter() - t0, v.kind,
+                                  v.start if v.converged else float(t[max(0, len(t) - WINDOW)])))
                 batch.remove(lat)
                 del active[key]
                 if log:
@@ -199,12 +201,31 @@ def run_sequential(cases: list[WingCase], lattices, storage: str = "f32", seed: 
                 if v.converged or t[-1] >= MAX_STEPS:
                     cd, cl = (v.drag, v.lift) if v.converged else [float(c[-WINDOW:].mean())
                                                                    for c in coefficients(case, f)]
-                    done.append(Solve(case, int(t[-1]), cd, cl, v.converged, time.perf_counter() - t0, v.kind))
+                    done.append(Solve(case, int(t[-1]), cd, cl, v.converged, time.perf_counter() - t0, v.kind,
+                                      v.start if v.converged else float(t[max(0, len(t) - WINDOW)])))
                     if log:
                         log(f"sequential: case {case.index} done at step {int(t[-1])}")
                     break
         s.close()
     return done, time.perf_counter() - t0
+
+
+def fp64_coefficients(case: WingCase, lattice, steps: int, mean_from: float, seed: int = 7) -> tuple[float, float]:
+    """The case's lattice alone in FP64 storage (the device FP64 oracle) from the same seeded
+    start to `steps`, its (Cd, Cl) averaged over the samples at step >= `mean_from`."""
+    from . import history as hist
+    config, mask = lattice
+    s = Lbm3DSolver(config, mask, "f64")
+    s.initialize_perturbed(seed, 1e-4)
+    times, forces = [], []
+    while s.steps() < steps:
+        s.step(SAMPLE_EVERY)
+        times.append(float(s.steps()))
+        forces.append(s.force().force)
+    s.close()
+    cd, cl = coefficients(case, np.array(forces))
+    t = np.array(times)
+    return hist.mean_since(t, cd, mean_from), hist.mean_since(t, cl, mean_from)
 
 
 def records(solves: list[Solve], storage: str = "f32") -> list:
@@ -245,4 +266,4 @@ def lift_slope(solves: list[Solve], attached=(-4.0, 8.0)) -> float:
 
 
 __all__ = ["WingCase", "wing_cases", "wing_lattice", "place_body", "run_batched", "run_sequential", "lift_slope",
-           "coefficients", "converged", "judge", "Solve", "records", "comparison_csv"]
+           "coefficients", "converged", "judge", "Solve", "records", "comparison_csv", "fp64_coefficients"]
