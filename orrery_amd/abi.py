@@ -10,7 +10,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY = os.path.join(HERE, "liborrery_lbm.so")
+# ORRERY_LBM_LIBRARY names another build of the same library (the host-sanitizer build of the
+# CPU tests); by default the in-tree build.
+LIBRARY = os.environ.get("ORRERY_LBM_LIBRARY") or os.path.join(HERE, "liborrery_lbm.so")
 
 LBM_OK, LBM_EINVAL, LBM_ENOMEM, LBM_EDEVICE, LBM_ESTATE = 0, 1, 2, 3, 4
 STORAGE = {"f32": 0, "f16s": 1, "bf16s": 2, "f64": 3}

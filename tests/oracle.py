@@ -14,7 +14,7 @@ from orrery_amd.config import Lbm2DConfig, Lbm3DConfig
 from orrery_amd.solver import BoundaryLinks, Force2D, Force3D, MacroFields2D, MacroFields3D
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIBRARY = os.path.join(ROOT, "oracle", "liborrery_oracle.so")
+LIBRARY = os.environ.get("ORRERY_ORACLE_LIBRARY") or os.path.join(ROOT, "oracle", "liborrery_oracle.so")
 
 P = ctypes.POINTER
 _f64p = P(ctypes.c_double)
