@@ -210,7 +210,15 @@ def batched_section(n_lattices, sequential_every, fp64_checks, storage="f32"):
     cases = polar.wing_cases(n_lattices)
     log(f"batched: voxelising {len(cases)} wing sections on the device")
     lattices = [polar.wing_lattice(c, device=True) for c in cases]
-    batched, tb = polar.run_batched(cases, lattices, storage, log=log)
+    timing = {}
+    batched, tb = polar.run_batched(cases, lattices, storage, log=log, timing=timing)
+    kernels = {"step_ms": round(timing["step_ms"], 4), "force_sample_ms": round(timing["force_ms"], 4)}
+    for st, g in timing["groups"].items():
+        sb = {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[st]
+        kernels[st] = {"bulk_ms": round(g["bulk_ms"], 4), "boundary_ms": round(g["boundary_ms"], 4),
+                       "bulk_nodes": g["bulk_nodes"], "boundary_nodes": g["boundary_nodes"],
+                       "bulk_gbps_algorithmic": round(38 * sb * g["bulk_nodes"] / (g["bulk_ms"] * 1e-3) / 1e9, 1),
+                       "bulk_frac_of_8tbs": round(38 * sb * g["bulk_nodes"] / (g["bulk_ms"] * 1e-3) / HBM_PEAK, 4)}
     pick = list(range(0, len(cases), max(1, sequential_every)))
     log(f"batched: one-at-a-time run of {len(pick)} cases")
     sequential, ts = polar.run_sequential([cases[k] for k in pick], [lattices[k] for k in pick], storage, log=log)
@@ -236,6 +244,7 @@ def batched_section(n_lattices, sequential_every, fp64_checks, storage="f32"):
         "workload": f"{len(cases)} NACA 0012 sections, dims in [96^3, 320x160x160], chord 48-96 cells, AoA -4..18 deg, "
                     f"Re 500-2000, LES, interpolated wall, periodic span, {storage}",
         "cells": sum(c.cells for c in cases), "converged": len(done), "converged_kinds": kinds,
+        "kernels_first_chunk": kernels,
         "batched_s": round(tb, 2), "converged_solves_per_hour": round(len(done) / tb * 3600.0, 1),
         "sequential_s_estimated": round(seq_total, 2), "sequential_cases_timed": len(pick),
         "sequential_converged_solves_per_hour": round(len(done) / seq_total * 3600.0, 1),
@@ -744,10 +753,11 @@ def main(argv=None):
         if storage == "f64" and compare:
             ref = {"force_x": results[storage]["force_x"], "fields": results[storage].pop("fields")}
     ref = None
-    main_r = results["f32"]
+    main_key = "f32" if "f32" in results else next(k for k in results if k != "f64")
+    main_r = results[main_key]
     cells = config.cells
     glups = cells / (main_r["ms_per_step"] * 1e-3) / 1e9
-    bytes_per_node = 2 * 19 * 4
+    bytes_per_node = 2 * 19 * {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[main_key]
     fused = main_r["launch"]["schedule"] == "fused"
     if fused:
         # one kernel per step: its time is the step time, over every node
@@ -771,12 +781,12 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": main_key,
         "data": "synthetic",
         "config": {
             "workload": "D3Q19 512x256x256 channel, voxelised Apollo-type capsule D64 pitched 20 deg (Re 1000, "
                         "u 0.05, Smagorinsky C 0.17), velocity inlet, convective outlet, slip sides, halfway "
-                        "bounce-back, FP32 A-A in place",
+                        f"bounce-back, {main_key} A-A in place",
             "cells": cells,
             "bulk_nodes": main_r["bulk_nodes"],
             "boundary_nodes": main_r["boundary_nodes"],
@@ -793,7 +803,8 @@ def main(argv=None):
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "traffic_over_algorithmic": round(traffic["bytes_per_launch"] / (bytes_per_node * main_r["bulk_nodes"]), 4)
             if traffic else None,
-            "kernel": "d3q19_fused<StoreF32>" if fused else "d3q19_bulk<StoreF32>",
+            "kernel": ("d3q19_fused<%s>" if fused else "d3q19_bulk<%s>") % {"f32": "StoreF32", "f16s": "StoreF16",
+                                                                            "bf16s": "StoreBF16", "f64": "StoreF64"}[main_key],
             "launch": main_r["launch"],
             "bulk_ms": round(main_r["bulk_ms"], 4),
             "boundary_ms": round(main_r["boundary_ms"], 4),

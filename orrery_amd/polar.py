@@ -145,10 +145,11 @@ class Solve:
 
 def run_batched(cases: list[WingCase], lattices, storage: str = "f32", budget_bytes: int = 240 * 10 ** 9,
                 chunk: int = WINDOW * SAMPLE_EVERY, seed: int = 7, log=None,
-                tune_steps: int = 10) -> tuple[list[Solve], float]:
+                tune_steps: int = 10, timing: dict | None = None) -> tuple[list[Solve], float]:
     """Every case resident in one batch (admission and, with `tune_steps`, the launch autotuner
     included in the time), advanced `chunk` steps per launch sequence; converged cases leave
-    the batch. `lattices` holds the (config, mask) of each case. Returns the solves and the wall
+    the batch. With `timing`, the first chunk runs with per-kernel HIP events and the dict
+    receives their times (LatticeBatch.run_timed). `lattices` holds the (config, mask) of each case. Returns the solves and the wall
     time."""
     t0 = time.perf_counter()
     batch = LatticeBatch(budget_bytes=budget_bytes, sample_every=SAMPLE_EVERY)
@@ -160,8 +161,14 @@ def run_batched(cases: list[WingCase], lattices, storage: str = "f32", budget_by
         if log:
             log(f"batched: launch shapes {chosen}")
     done = []
+    first = True
     while active:
-        batch.run(chunk)
+        if first and timing is not None:
+            # the first chunk with each kernel between HIP events (the same results as run())
+            timing.update(batch.run_timed(chunk))
+        else:
+            batch.run(chunk)
+        first = False
         for key in list(active):
             case, lat = active[key]
             t, f = lat.history()
