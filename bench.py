@@ -768,7 +768,7 @@ def main(argv=None):
     if args.cpu_steps > 0:
         print("bench: timing the CPU baseline (oracle)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(config, mask, args.cpu_steps, args.cpu_seconds)
-    traffic = measured_traffic()
+    traffic = measured_traffic() if main_key == "f32" else None  # the counter runs cover the f32 kernel
     line = {
         "metric": "D3Q19 GLUPS (FP32 A-A, BGK+Smagorinsky) and % HBM roofline; batched Cd solves/hour, p50/p99 "
                   "latency @1% Cd error",
