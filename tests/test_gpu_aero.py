@@ -160,3 +160,43 @@ def test_transient_is_not_converged_and_a_steady_wake_is():
     r = s.run()
     print(f"measured: steady wake cd {r.cd} converged {r.converged}")
     assert r.converged and not r.has_warning("lbm.unconverged")
+
+
+def test_wall_pressure_against_the_oracle_moments():
+    """The device-sampled wall Cp and largest speed of the FP64 lattice against the oracle's
+    moments (macroscopic(), lbm3d_cpu.cpp:154-186) of the same state: the device populations
+    are handed to the oracle and its moments give the expected samples."""
+    import ctypes
+    from oracle import Oracle3D
+    from orrery_amd import abi
+    s = LbmAeroSolver3D(coarse_options(duration=1.0, storage="f64"))
+    s.setup(sphere_geometry(0.02), flow_at(20.0, 0.02))
+    r = s.run()
+    lib = abi.load()
+    c = s.info().config
+    f = np.empty(19 * c.cells)
+    assert lib.lbm3d_populations(s.lattice_handle(), abi.ptr(f, ctypes.c_double)) == 0
+    cpu = Oracle3D(c, s.mask())
+    cpu.set_populations(f)
+    m = cpu.macroscopic()
+    rho = m.density.reshape(c.nz, c.ny, c.nx)
+    rho_inf = 0.0
+    for z in range(c.nz):
+        for y in range(c.ny):
+            rho_inf += rho[z, y, 1]
+    rho_inf /= c.ny * c.nz
+    u = s.info().lattice_velocity
+    n = ctypes.c_size_t()
+    lib.lbm3d_links(s.lattice_handle(), None, None, None, 0, ctypes.byref(n))
+    cells = np.empty(n.value, dtype=np.uint32)
+    lib.lbm3d_links(s.lattice_handle(), abi.ptr(cells, ctypes.c_uint32), None, None, n.value, ctypes.byref(n))
+    wall = np.unique(cells)
+    want = (1.0 / 3.0) * (m.density[wall] - rho_inf) / (0.5 * u * u)
+    worst = float(np.max(np.abs(r.surface.cp - want)))
+    print(f"measured: wall Cp vs oracle moments: max |d| {worst:.3e} over {len(wall)} samples", flush=True)
+    assert worst < 1e-12
+    speed = ctypes.c_double()
+    assert lib.lbm3d_max_speed(s.lattice_handle(), ctypes.byref(speed)) == 0
+    fluid = s.mask().solid == 0
+    ref = np.sqrt(m.velocity_x ** 2 + m.velocity_y ** 2 + m.velocity_z ** 2)[fluid].max()
+    assert speed.value == pytest.approx(ref, rel=1e-14)
