@@ -255,6 +255,7 @@ def batched_section(n_lattices, sequential_every, fp64_checks, storage="f32"):
         "lift_slope_per_rad": round(polar.lift_slope(batched), 4),
         "cd_vs_fp64": errors,
         "cd_rel_error_vs_fp64_max": max(e["cd_rel_error"] for e in errors) if errors else None,
+        "cd_within_1pct_of_fp64": f"{sum(e['cd_rel_error'] <= 0.01 for e in errors)}/{len(errors)}",
     }
 
 
@@ -315,6 +316,7 @@ def served_section(n_requests, rate, fp64_checks, sequential_every):
         "glups_served": round(work_all / span / 1e9, 2), "glups_sequential": round(work_pick / seq_stepping / 1e9, 2),
         "cd_vs_fp64": errors,
         "cd_rel_error_vs_fp64_max": max(e["cd_rel_error"] for e in errors) if errors else None,
+        "cd_within_1pct_of_fp64": f"{sum(e['cd_rel_error'] <= 0.01 for e in errors)}/{len(errors)}",
     }
 
 
