@@ -222,9 +222,13 @@ def batched_section(n_lattices, sequential_every, fp64_checks, storage="f32"):
     pick = list(range(0, len(cases), max(1, sequential_every)))
     log(f"batched: one-at-a-time run of {len(pick)} cases")
     sequential, ts = polar.run_sequential([cases[k] for k in pick], [lattices[k] for k in pick], storage, log=log)
+    log(f"batched: the same {len(pick)} cases on the reference adapter's schedule (full field every 200 steps)")
+    _, ts_ref = polar.run_sequential([cases[k] for k in pick], [lattices[k] for k in pick], storage, log=log,
+                                     field_every=200)
     work = {s.case.index: s.case.cells * s.steps for s in batched}
     share = sum(work[k] for k in pick) / sum(work.values())
     seq_total = ts / share
+    seq_ref_total = ts_ref / share
     seq_glups = sum(work[k] for k in pick) / ts / 1e9
     by_case = {s.case.index: s for s in batched}
     same = all((s.steps, s.cd, s.cl) == (by_case[s.case.index].steps, by_case[s.case.index].cd,
@@ -250,6 +254,8 @@ def batched_section(n_lattices, sequential_every, fp64_checks, storage="f32"):
         "sequential_converged_solves_per_hour": round(len(done) / seq_total * 3600.0, 1),
         "speedup": round(seq_total / tb, 3), "same_results_as_sequential": same,
         "glups_batched": round(sum(work.values()) / tb / 1e9, 2), "glups_sequential": round(seq_glups, 2),
+        "sequential_reference_schedule_s_estimated": round(seq_ref_total, 2),
+        "speedup_vs_reference_schedule": round(seq_ref_total / tb, 3),
         "sequential_fixed_duration_s_estimated": round(plain_fixed, 1),
         "speedup_vs_fixed_duration": round(plain_fixed / tb, 3),
         "lift_slope_per_rad": round(polar.lift_slope(batched), 4),

@@ -187,9 +187,12 @@ def run_batched(cases: list[WingCase], lattices, storage: str = "f32", budget_by
 
 
 def run_sequential(cases: list[WingCase], lattices, storage: str = "f32", seed: int = 7,
-                   log=None) -> tuple[list[Solve], float]:
+                   log=None, field_every: int = 0) -> tuple[list[Solve], float]:
     """One plain lattice at a time (its creation included in the time), its force read back
-    every sample (host-paced)."""
+    every sample (host-paced). With `field_every`, the whole macroscopic field is also read
+    back every `field_every` steps, as the reference's adapter does on every step() call
+    (LbmAeroSolver3D::step, src/aero/lbm3d/lbm_aero_solver3d.cpp:387-490; 200 steps per call by
+    default)."""
     done = []
     t0 = time.perf_counter()
     for case, (config, mask) in zip(cases, lattices):
@@ -201,6 +204,8 @@ def run_sequential(cases: list[WingCase], lattices, storage: str = "f32", seed: 
             s.step(SAMPLE_EVERY)
             times.append(float(s.steps()))
             forces.append(s.force().force)
+            if field_every > 0 and s.steps() % field_every == 0:
+                s.macroscopic()
             if len(times) % WINDOW == 0:
                 t = np.array(times)
                 f = np.array(forces)
