@@ -2,35 +2,21 @@
 Lbm3DSolver interface over the C ABI. Compiled against the reference's own headers where the
 reference tree is present; its refusals carry the reference's validate() messages, compared
 with the reference's validate compiled from its sources (oracle/_ref)."""
-import os
-import shutil
 import subprocess
 
 import pytest
 
+import adapter_build
 import refcheck
 from orrery_amd import Lbm3DConfig, Wall, WallKind, SolidWall, fluid_mask_3d
-from orrery_amd.abi import LIBRARY
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-ROOT = os.path.dirname(HERE)
-REFERENCE = "/root/reference"
-INCLUDES = ["src/aero/lbm3d/include", "src/aero/lbm/include", "src/aero/compute/include", "src/core/include",
-            "src/geom/include", "src/aero/include"]
-
-pytestmark = pytest.mark.skipif(not (os.path.isdir(REFERENCE) and shutil.which("g++") and refcheck.available()),
+pytestmark = pytest.mark.skipif(not (adapter_build.possible() and refcheck.available()),
                                 reason="needs the reference's headers and oracle/_ref")
 
 
 @pytest.fixture(scope="module")
 def driver(tmp_path_factory):
-    exe = str(tmp_path_factory.mktemp("adapter") / "adapter_check")
-    cmd = ["g++", "-std=c++20", "-O1", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include")]
-    for inc in INCLUDES:
-        cmd += ["-I", os.path.join(REFERENCE, inc)]
-    cmd += [os.path.join(HERE, "adapter_check.cpp"), "-o", exe, LIBRARY, f"-Wl,-rpath,{os.path.dirname(LIBRARY)}"]
-    subprocess.run(cmd, check=True, capture_output=True, text=True)
-    return exe
+    return adapter_build.build(str(tmp_path_factory.mktemp("adapter") / "adapter_check"))
 
 
 def config(what):
