@@ -8,7 +8,9 @@ a velocity inlet, a convective outlet and slip sides, around a voxelised Apollo-
 from a seeded +-1e-4 perturbation of the uniform inlet state. A step is one lattice update of
 every node (bulk kernel + boundary kernel). `value` = nx ny nz K / time in GLUPS, timed with HIP
 events on the solver's stream after W warm-up steps. The roofline object prices the bulk kernel
-(the dominant one) at 152 algorithmic bytes per bulk node against MI355X's 8 TB/s; cpu_baseline is
+(the dominant one) at 152 algorithmic bytes per node it updates (bulk nodes, and in A steps the
+side nodes of the slip faces, which a side kernel updates in B steps) against MI355X's 8 TB/s;
+cpu_baseline is
 the oracle (the FP64 CPU restatement of the reference solver) on a bounded number of steps of the
 same lattice on all host cores. Each storage format (f32, f16s, bf16s) is also run on to step
 2000 and its drag force and velocity field compared with the f64 storage's (the FP64 oracle
@@ -90,27 +92,33 @@ def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare
         sched, block, cpt = autotune.current(solver)
         launch = {"schedule": sched, "block": block, "cells_per_thread": cpt, "groups_per_cu": list(solver.occupancy())}
     ms = ctypes.c_double()
-    bulk = ctypes.c_double()
-    bnd = ctypes.c_double()
     rc = lib.lbm3d_time_steps(solver.handle, steps, 0, ctypes.byref(ms), None, None)
     if rc:
         raise RuntimeError(lib.lbm_last_error().decode())
     total_ms = ms.value
-    rc = lib.lbm3d_time_steps(solver.handle, per_kernel_steps, 1, ctypes.byref(ms), ctypes.byref(bulk),
-                              ctypes.byref(bnd))
-    if rc:
+    kernels = (ctypes.c_double * 6)()
+    if lib.lbm3d_time_kernels(solver.handle, per_kernel_steps, kernels):
         raise RuntimeError(lib.lbm_last_error().decode())
     counts = [ctypes.c_size_t() for _ in range(4)]
     lib.lbm3d_counts(solver.handle, *[ctypes.byref(c) for c in counts])
+    split = stream_counts(lib, solver.handle)
     out = {
         "launch": launch,
         "ms_per_step": total_ms / steps,
-        "bulk_ms": bulk.value / per_kernel_steps,
-        "boundary_ms": bnd.value / per_kernel_steps,
+        # mean over A and B steps of each kernel; the side kernel runs in B steps only
+        "bulk_ms": (kernels[0] + kernels[1]) / 2,
+        "bulk_ms_a_b": [round(kernels[0], 4), round(kernels[1], 4)],
+        "side_ms_b": kernels[3],
+        "boundary_ms": (kernels[4] + kernels[5]) / 2,
         "bulk_nodes": counts[0].value,
         "boundary_nodes": counts[1].value,
         "solid_nodes": counts[2].value,
         "links": counts[3].value,
+        # the bulk kernel updates bulk and side nodes in A steps and bulk nodes in B steps; the
+        # side kernel side nodes in B steps; the boundary kernel the listed boundary nodes
+        "bulk_kernel_nodes": split["bulk"] + split["side"] / 2,
+        "side_nodes": split["side"],
+        "listed_nodes": split["listed"],
     }
     if compare_steps > 0:
         solver.step(max(0, compare_steps - solver.steps()))
@@ -129,6 +137,15 @@ def time_storage(config, mask, storage, steps, warmup, per_kernel_steps, compare
                                                       (ref["fields"] ** 2).sum()))
     solver.close()
     return out
+
+
+def stream_counts(lib, handle):
+    """lbm3d_stream_counts: nodes the bulk kernel streams (bulk, side) and the boundary kernel's."""
+    import ctypes
+    v = [ctypes.c_size_t() for _ in range(3)]
+    if lib.lbm3d_stream_counts(handle, *[ctypes.byref(x) for x in v]):
+        raise RuntimeError(lib.lbm_last_error().decode())
+    return {"bulk": v[0].value, "side": v[1].value, "listed": v[2].value}
 
 
 def cpu_baseline(config, mask, steps, seconds=0.0):
@@ -495,21 +512,27 @@ def sphere_bench(args):
             cd.append(s.force().force[0] / q)
         wall = time.perf_counter() - t0
         t, cd = np.array(t), np.array(cd)
-        ms, bulk, bnd = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        ms = ctypes.c_double()
         for rc in (lib.lbm3d_time_steps(s.handle, args.steps, 0, ctypes.byref(ms), None, None),):
             if rc:
                 raise RuntimeError(lib.lbm_last_error().decode())
         step_ms = ms.value / args.steps
-        if lib.lbm3d_time_steps(s.handle, 100, 1, ctypes.byref(ms), ctypes.byref(bulk), ctypes.byref(bnd)):
+        kernels = (ctypes.c_double * 6)()
+        if lib.lbm3d_time_kernels(s.handle, 100, kernels):
             raise RuntimeError(lib.lbm_last_error().decode())
         counts = [ctypes.c_size_t() for _ in range(4)]
         lib.lbm3d_counts(s.handle, *[ctypes.byref(c) for c in counts])
+        split = stream_counts(lib, s.handle)
+        per_launch = split["bulk"] + split["side"] / 2
+        bulk_ms = (kernels[0] + kernels[1]) / 2
         sb = 4 if storage == "f32" else 8
         out[storage] = {"cd": float(history.mean_since(t, cd, float(steps - 1000))), "cd_last": float(cd[-1]),
                         "glups": config.cells / (step_ms * 1e-3) / 1e9, "ms_per_step": step_ms,
-                        "sampled_run_s": wall, "bulk_ms": bulk.value / 100, "boundary_ms": bnd.value / 100,
-                        "bulk_gbps": 2 * 19 * sb * counts[0].value / (bulk.value / 100 * 1e-3) / 1e9,
-                        "bulk_nodes": counts[0].value, "solid_nodes": counts[2].value, "links": counts[3].value}
+                        "sampled_run_s": wall, "bulk_ms": bulk_ms, "side_ms_b_step": kernels[3],
+                        "boundary_ms": (kernels[4] + kernels[5]) / 2,
+                        "bulk_gbps": 2 * 19 * sb * per_launch / (bulk_ms * 1e-3) / 1e9,
+                        "bulk_nodes": counts[0].value, "side_nodes": split["side"], "solid_nodes": counts[2].value,
+                        "links": counts[3].value}
         s.close()
     out["f32"]["cd_rel_error_vs_f64"] = abs(out["f32"]["cd"] / out["f64"]["cd"] - 1.0)
     cpu = None
@@ -771,7 +794,7 @@ def main(argv=None):
         # one kernel per step: its time is the step time, over every node
         achieved = bytes_per_node * cells / (main_r["ms_per_step"] * 1e-3)
     else:
-        achieved = bytes_per_node * main_r["bulk_nodes"] / (main_r["bulk_ms"] * 1e-3)
+        achieved = bytes_per_node * main_r["bulk_kernel_nodes"] / (main_r["bulk_ms"] * 1e-3)
     cpu = None
     if args.cpu_steps > 0:
         print("bench: timing the CPU baseline (oracle)", file=sys.stderr, flush=True)
@@ -798,6 +821,8 @@ def main(argv=None):
             "cells": cells,
             "bulk_nodes": main_r["bulk_nodes"],
             "boundary_nodes": main_r["boundary_nodes"],
+            "side_nodes": main_r["side_nodes"],
+            "listed_boundary_nodes": main_r["listed_nodes"],
             "solid_nodes": main_r["solid_nodes"],
             "links": main_r["links"],
             "parallelism": "single GPU",
@@ -809,12 +834,15 @@ def main(argv=None):
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
             "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "traffic_over_algorithmic": round(traffic["bytes_per_launch"] / (bytes_per_node * main_r["bulk_nodes"]), 4)
+            "traffic_over_algorithmic": round(traffic["bytes_per_launch"] / (bytes_per_node * main_r["bulk_kernel_nodes"]), 4)
             if traffic else None,
             "kernel": ("d3q19_fused<%s>" if fused else "d3q19_bulk<%s>") % {"f32": "StoreF32", "f16s": "StoreF16",
                                                                             "bf16s": "StoreBF16", "f64": "StoreF64"}[main_key],
             "launch": main_r["launch"],
             "bulk_ms": round(main_r["bulk_ms"], 4),
+            "bulk_ms_a_b": main_r["bulk_ms_a_b"],
+            "nodes_per_launch": main_r["bulk_kernel_nodes"],
+            "side_ms_b_step": round(main_r["side_ms_b"], 4),
             "boundary_ms": round(main_r["boundary_ms"], 4),
             "step_roofline_frac": round(bytes_per_node * cells / (main_r["ms_per_step"] * 1e-3) / HBM_PEAK, 4),
         },
@@ -824,7 +852,7 @@ def main(argv=None):
         "storage_variants": {
             k: {"glups": round(cells / (r["ms_per_step"] * 1e-3) / 1e9, 3),
                 "bytes_per_node": 2 * 19 * {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[k],
-                "hbm_frac_bulk": round(2 * 19 * {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[k] * r["bulk_nodes"]
+                "hbm_frac_bulk": round(2 * 19 * {"f32": 4, "f16s": 2, "bf16s": 2, "f64": 8}[k] * r["bulk_kernel_nodes"]
                                        / (r["bulk_ms"] * 1e-3) / HBM_PEAK, 4),
                 "ms_per_step": round(r["ms_per_step"], 4), "launch": r["launch"],
                 **{e: float(f"{r[e]:.3e}") for e in ("force_x_rel_error_vs_f64", "velocity_l2_vs_f64") if e in r},

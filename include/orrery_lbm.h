@@ -210,17 +210,26 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
 /* Bytes one step of this lattice moves by the algorithmic count: 2 * 19 * bytes per stored
  * population, times nx * ny * nz (the reference's MLUPS convention). */
 int lbm3d_step_bytes(const lbm3d* h, double* bytes);
-/* Node counts of the lattice: bulk (plain streaming), boundary, solid, and boundary links. */
+/* Node counts of the lattice: bulk (plain streaming), boundary, solid, and boundary links, as the
+ * reference classifies them (classify_nodes, lbm3d_internal.h:219-248). */
 int lbm3d_counts(const lbm3d* h, size_t* bulk, size_t* boundary, size_t* solid, size_t* links);
+/* No counterpart in the reference (its kernels do not split the boundary nodes): how this
+ * library's step kernels divide the fluid nodes. `bulk` and `side` nodes are streamed in place by
+ * the bulk kernel (side nodes: the boundary nodes on slip or periodic y and z faces, away from the
+ * x faces and from solids); `listed` nodes by the boundary kernel, through source codes and the
+ * stash. bulk + side + listed is the number of fluid nodes. */
+int lbm3d_stream_counts(const lbm3d* h, size_t* bulk, size_t* side, size_t* listed);
 /* Advances `count` steps and times them with HIP events on the handle's stream (the
  * reference's bench protocol, tests/bench/aero/bench_lbm_gpu.cpp:54-64). With per_kernel = 0,
- * one event pair brackets all steps (*step_ms); with per_kernel = 1, an event pair also
- * brackets every launch, giving the summed durations of the bulk and boundary kernels. */
+ * one event pair brackets all steps (*step_ms); with per_kernel = 1, event pairs also bracket
+ * the bulk kernel and the rest of each step (side and boundary kernels), giving their summed
+ * durations in *bulk_ms and *boundary_ms. */
 int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_ms, double* bulk_ms,
                      double* boundary_ms);
 /* Advances `count` steps with HIP events around each kernel on the handle's stream: ms[0], ms[1]
- * the bulk kernel's mean duration in A and B steps, ms[2], ms[3] the boundary kernel's. */
-int lbm3d_time_kernels(lbm3d* h, uint64_t count, double ms[4]);
+ * the bulk kernel's mean duration in A and B steps, ms[2], ms[3] the side kernel's (it runs in B
+ * steps only: ms[2] = 0), ms[4], ms[5] the boundary kernel's. */
+int lbm3d_time_kernels(lbm3d* h, uint64_t count, double ms[6]);
 
 /* ---- D2Q9 (same shape; lbm2d.h:173-211, lbm2d_gpu.cpp:68-378) ---------------------------- */
 int lbm2d_create(const lbm2d_config* config, const uint8_t* solid, int32_t storage, lbm2d** out);
@@ -298,11 +307,11 @@ int lbm_batch_set_inlet(lbm_batch* b, int32_t id, const double velocity[3]);
  * where per storage group. */
 typedef struct lbm_batch_timing {
     double step_ms;          /* wall time of the run / steps (force samples included) */
-    double bulk_ms[4];       /* bulk kernel of the group, mean per step (0: no such group) */
+    double bulk_ms[4];       /* bulk and side kernels of the group, mean per step (0: no such group) */
     double boundary_ms[4];   /* boundary kernel of the group, mean per step */
     double force_ms;         /* force kernels of every group, mean per sample */
-    uint64_t bulk_nodes[4];  /* bulk nodes the group's bulk launch updates */
-    uint64_t boundary_nodes[4];
+    uint64_t bulk_nodes[4];  /* bulk and side nodes the group's bulk launch updates */
+    uint64_t boundary_nodes[4]; /* listed boundary nodes the group's boundary launch updates */
 } lbm_batch_timing;
 /* lbm_batch_run with every kernel launched on its own (no graph replay) between HIP events:
  * the same kernels in the same order, so the lattices and their histories advance exactly as

@@ -164,6 +164,7 @@ EXPORTS = [
     ("lbm3d_get_occupancy", ctypes.c_int, [ctypes.c_void_p, P(ctypes.c_int32), P(ctypes.c_int32)]),
     ("lbm3d_step_bytes", ctypes.c_int, [ctypes.c_void_p, c_f64p]),
     ("lbm3d_counts", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep, c_sizep, c_sizep]),
+    ("lbm3d_stream_counts", ctypes.c_int, [ctypes.c_void_p, c_sizep, c_sizep, c_sizep]),
     ("lbm3d_time_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, c_f64p, c_f64p, c_f64p]),
     ("lbm2d_create", ctypes.c_int, [P(CConfig2D), c_u8p, ctypes.c_int32, P(ctypes.c_void_p)]),
     ("lbm2d_destroy", ctypes.c_int, [ctypes.c_void_p]),

@@ -127,6 +127,7 @@ template <class T> hipError_t step_once(lbm3d& h, const lbm::Lattice3<typename T
         if (e == hipSuccess) e = hipStreamWaitEvent(h.side, h.fork, 0);
         if (e == hipSuccess) e = lbm::d3q19_step_boundary<T>(p, local, h.side);
         if (e == hipSuccess) e = lbm::d3q19_step_bulk<T>(p, local, h.block, local ? h.cpt : h.cpt_b, bulk_lds(h, local), h.stream);
+        if (e == hipSuccess) e = lbm::d3q19_step_side<T>(p, local, h.stream);
         if (e == hipSuccess) e = hipEventRecord(h.join, h.side);
         if (e == hipSuccess) e = hipStreamWaitEvent(h.stream, h.join, 0);
     }
@@ -287,6 +288,8 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
         ((h->bcell = static_cast<std::uint32_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, t.tile_slot.size() * 4U, "boundary slot table")) ||
         ((h->tile_slot = static_cast<std::uint32_t*>(tmp)), false) ||
+        (rc = alloc(*h, &tmp, std::max<std::size_t>(t.scell.size(), 1) * 4U, "side list")) ||
+        ((h->scell = static_cast<std::uint32_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, 19U * static_cast<std::size_t>(t.bstride) * 4U, "source codes")) ||
         ((h->bcode = static_cast<std::uint32_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, static_cast<std::size_t>(t.bstride) * 4U, "lattice write masks")) ||
@@ -328,6 +331,7 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     }
     if ((rc = upload(*h, h->cls, t.cls, "node class upload")) || (rc = upload(*h, h->bcell, t.bcell, "boundary upload")) ||
         (rc = upload(*h, h->tile_slot, t.tile_slot, "boundary slot upload")) ||
+        (rc = upload(*h, h->scell, t.scell, "side list upload")) ||
         (rc = upload(*h, h->bcode, t.bcode, "code upload")) || (rc = upload(*h, h->bwrite, t.bwrite, "mask upload")) ||
         (rc = upload(*h, h->lcell, lcell, "link upload")) ||
         (rc = upload(*h, h->ldir, ldir, "link upload")) || (rc = upload(*h, h->lq, lq, "link upload")) ||
@@ -859,14 +863,28 @@ int lbm3d_counts(const lbm3d* h, size_t* bulk, size_t* boundary, size_t* solid, 
     if (h == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null handle");
     }
-    std::size_t n[3] = {0, 0, 0};
+    std::size_t n[4] = {0, 0, 0, 0};
     for (const std::uint8_t c : h->tables.cls) {
-        ++n[c];
+        ++n[c & 3U];
     }
     if (bulk != nullptr) *bulk = n[1];
-    if (boundary != nullptr) *boundary = n[2];
+    if (boundary != nullptr) *boundary = n[2] + n[3];
     if (solid != nullptr) *solid = n[0];
     if (links != nullptr) *links = h->tables.links.size();
+    return ok();
+}
+
+int lbm3d_stream_counts(const lbm3d* h, size_t* bulk, size_t* side, size_t* listed) {
+    if (h == nullptr) {
+        return fail(LBM_EINVAL, "lbm3d: null handle");
+    }
+    std::size_t n[4] = {0, 0, 0, 0};
+    for (const std::uint8_t c : h->tables.cls) {
+        ++n[c & 3U];
+    }
+    if (bulk != nullptr) *bulk = n[1];
+    if (side != nullptr) *side = n[3];
+    if (listed != nullptr) *listed = n[2];
     return ok();
 }
 
@@ -904,6 +922,9 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
                 }
                 if (err == hipSuccess && per_kernel) {
                     err = hipEventRecord(ev[3U * n + 2U], h->stream);
+                }
+                if (err == hipSuccess) {
+                    err = lbm::d3q19_step_side<T>(p, local, h->stream);
                 }
                 if (err == hipSuccess) {
                     err = lbm::d3q19_step_boundary<T>(p, local, h->stream);
@@ -955,15 +976,16 @@ int lbm3d_time_steps(lbm3d* h, uint64_t count, int32_t per_kernel, double* step_
     return rc;
 }
 
-int lbm3d_time_kernels(lbm3d* h, uint64_t count, double ms[4]) {
+int lbm3d_time_kernels(lbm3d* h, uint64_t count, double ms[6]) {
     if (h == nullptr || ms == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null argument");
     }
     if (h->in_batch) {
         return refuse_in_batch("time_kernels");
     }
-    // Marks around each kernel: [bulk start, bulk end = boundary start, boundary end] per step.
-    std::vector<hipEvent_t> ev(3U * count, nullptr);
+    // Marks around each kernel: [bulk start, bulk end = side start, side end = boundary start,
+    // boundary end] per step.
+    std::vector<hipEvent_t> ev(4U * count, nullptr);
     int rc = LBM_OK;
     for (auto& e : ev) {
         if (hipEventCreate(&e) != hipSuccess) {
@@ -975,14 +997,16 @@ int lbm3d_time_kernels(lbm3d* h, uint64_t count, double ms[4]) {
         const bool local = (h->steps & 1U) != 0;
         rc = with_store(*h, [&]<class T>() -> int {
             const auto p = descriptor<T>(*h);
-            hipError_t err = hipEventRecord(ev[3U * n], h->stream);
+            hipError_t err = hipEventRecord(ev[4U * n], h->stream);
             if (err == hipSuccess) {
                 err = lbm::d3q19_step_bulk<T>(p, local, h->block, local ? h->cpt : h->cpt_b, bulk_lds(*h, local),
                                               h->stream);
             }
-            if (err == hipSuccess) err = hipEventRecord(ev[3U * n + 1U], h->stream);
+            if (err == hipSuccess) err = hipEventRecord(ev[4U * n + 1U], h->stream);
+            if (err == hipSuccess) err = lbm::d3q19_step_side<T>(p, local, h->stream);
+            if (err == hipSuccess) err = hipEventRecord(ev[4U * n + 2U], h->stream);
             if (err == hipSuccess) err = lbm::d3q19_step_boundary<T>(p, local, h->stream);
-            if (err == hipSuccess) err = hipEventRecord(ev[3U * n + 2U], h->stream);
+            if (err == hipSuccess) err = hipEventRecord(ev[4U * n + 3U], h->stream);
             if (err == hipSuccess) err = end_step<T>(*h);
             return err == hipSuccess ? ok() : hip_fail("timed kernels", err);
         });
@@ -991,26 +1015,25 @@ int lbm3d_time_kernels(lbm3d* h, uint64_t count, double ms[4]) {
         }
     }
     if (rc == LBM_OK && count > 0) {
-        const hipError_t e = hipEventSynchronize(ev[3U * count - 1U]);
+        const hipError_t e = hipEventSynchronize(ev[4U * count - 1U]);
         if (e != hipSuccess) {
             rc = hip_fail("timed kernels", e);
         }
     }
     if (rc == LBM_OK) {
-        double sum[4] = {0, 0, 0, 0};
+        double sum[6] = {0, 0, 0, 0, 0, 0};
         std::uint64_t launches[2] = {0, 0};
         std::uint64_t first = h->steps - count;
         for (std::uint64_t n = 0; n < count; ++n) {
             const int b_step = ((first + n) & 1U) == 0 ? 1 : 0; // even step count before: a B step
-            float a = 0.0f;
-            float b = 0.0f;
-            (void)hipEventElapsedTime(&a, ev[3U * n], ev[3U * n + 1U]);
-            (void)hipEventElapsedTime(&b, ev[3U * n + 1U], ev[3U * n + 2U]);
-            sum[b_step] += a;
-            sum[2 + b_step] += b;
+            for (int k = 0; k < 3; ++k) {
+                float t = 0.0f;
+                (void)hipEventElapsedTime(&t, ev[4U * n + k], ev[4U * n + k + 1U]);
+                sum[2 * k + b_step] += t;
+            }
             ++launches[b_step];
         }
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 6; ++k) {
             const std::uint64_t l = launches[k & 1];
             ms[k] = l > 0 ? sum[k] / static_cast<double>(l) : 0.0;
         }

@@ -37,18 +37,28 @@ __global__ __launch_bounds__(256) void d3q19_fused(Lattice3<typename T::R> p) {
     const bool inside = cell < p.cells;
     const std::uint32_t c = inside ? cell : p.cells - 1;
     const std::uint8_t kind = inside ? p.cls[c] : std::uint8_t(0);
-    {
+    if (kLocal || kind != kSideNode) {
         R v[19];
         d3::bulk_load<T, kLocal>(p, c, v);
-        if (kind == 1) {
-            d3::bulk_finish<T, kLocal>(p, c, v);
+        if (d3::streamed(kind)) {
+            d3::bulk_finish<T, kLocal>(p, c, v);  // in the A step a side node streams as a bulk node
         }
+    } else {
+        d3::side_node_b<T>(p, c);
     }
     const std::uint64_t mask = __ballot(kind == 2);
     if (kind == 2) {
         const std::uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<std::uint32_t>(mask >> 32),
                                                              __builtin_amdgcn_mbcnt_lo(static_cast<std::uint32_t>(mask), 0U));
         d3::boundary_node<T, kLocal>(p, p.tile_slot[c >> 6] + rank);
+    }
+}
+
+// B step of the side nodes (in A steps the bulk kernel updates them).
+template <class T> __global__ __launch_bounds__(256) void d3q19_side(Lattice3<typename T::R> p) {
+    const std::uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < p.ns) {
+        d3::side_node_b<T>(p, p.scell[s]);
     }
 }
 
@@ -145,19 +155,24 @@ __global__ __launch_bounds__(256) void d3q19_init_boundary(Lattice3<typename T::
 }
 
 // ---- readbacks -------------------------------------------------------------------------------
-// Post-collision population i of a bulk node in the layout of `phase` = n & 1.
+// Post-collision population i of a bulk or side node (class `kind`) in the layout of
+// `phase` = n & 1: after an A step its own cell's A[opp(i)]; after a B step the address its
+// reader loads, which is the B-step address of the node's pull opp(i).
 template <class T>
 __device__ __forceinline__ typename T::R bulk_value(const Lattice3<typename T::R>& p, const typename T::S* f,
-                                                    int i, std::uint32_t cell, int phase) {
+                                                    int i, std::uint32_t cell, int phase, std::uint8_t kind) {
     if (phase == 0) {
         return T::load(f[L3::opp(i) * p.stride + cell]);
+    }
+    if (kind == kSideNode) {
+        return T::load(f[d3::side_element(p, cell, d3::side_faces(p, cell), L3::opp(i), 0U)]);
     }
     const int off = off3(i, static_cast<int>(p.nx), static_cast<int>(p.nx * p.ny));
     return T::load(f[i * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off)]);
 }
 
 // Populations with the weights added back, [19][cells] doubles. Pass 1 over cells covers
-// solid and bulk nodes; pass 2 over the boundary list fills boundary nodes from the stash.
+// solid, bulk and side nodes; pass 2 over the boundary list fills boundary nodes from the stash.
 template <class T>
 __global__ __launch_bounds__(256) void d3q19_populations_cells(Lattice3<typename T::R> p, int phase, double* out) {
     const std::uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -165,17 +180,17 @@ __global__ __launch_bounds__(256) void d3q19_populations_cells(Lattice3<typename
         return;
     }
     const std::uint8_t c = p.cls[cell];
-    if (c == 2) {
+    if (c == kListedNode) {
         return;
     }
     const auto* f = static_cast<const typename T::S*>(p.f);
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         double v;
-        if (c == 0) {
+        if (c == kSolidNode) {
             v = L3::w(i);
         } else {
-            v = static_cast<double>(bulk_value<T>(p, f, i, cell, phase));
+            v = static_cast<double>(bulk_value<T>(p, f, i, cell, phase, c));
             if constexpr (T::shifted) {
                 v += L3::w(i);
             }
@@ -232,16 +247,16 @@ __global__ __launch_bounds__(256) void d3q19_moments_cells(Lattice3<typename T::
         return;
     }
     const std::uint8_t c = p.cls[cell];
-    if (c == 2) {
+    if (c == kListedNode) {
         return;
     }
     double rho = 1.0, ux = 0.0, uy = 0.0, uz = 0.0;
-    if (c == 1) {
+    if (d3::streamed(c)) {
         const auto* f = static_cast<const typename T::S*>(p.f);
         double v[19];
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-            v[i] = static_cast<double>(bulk_value<T>(p, f, i, cell, phase));
+            v[i] = static_cast<double>(bulk_value<T>(p, f, i, cell, phase, c));
         }
         const double force[3] = {fx, fy, fz};
         moments_of<T>(v, force, rho, ux, uy, uz);
@@ -368,17 +383,19 @@ __global__ __launch_bounds__(256) void d3q19_force(Lattice3<typename T::R> p, in
     block_sum<6>(v, partial);
 }
 
-// Mass: pass over cells (bulk nodes) and pass over the boundary list, each to block partials.
+// Mass: pass over cells (bulk and side nodes) and pass over the boundary list, each to block
+// partials.
 template <class T>
 __global__ __launch_bounds__(256) void d3q19_mass_cells(Lattice3<typename T::R> p, int phase, double* partial) {
     const std::uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
     double v[1] = {0.0};
-    if (cell < p.cells && p.cls[cell] == 1) {
+    const std::uint8_t c = cell < p.cells ? p.cls[cell] : std::uint8_t(kSolidNode);
+    if (d3::streamed(c)) {
         const auto* f = static_cast<const typename T::S*>(p.f);
         double s = 0.0;
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-            s += static_cast<double>(bulk_value<T>(p, f, i, cell, phase)) + (T::shifted ? L3::w(i) : 0.0);
+            s += static_cast<double>(bulk_value<T>(p, f, i, cell, phase, c)) + (T::shifted ? L3::w(i) : 0.0);
         }
         v[0] = s;
     }
@@ -402,7 +419,8 @@ __global__ __launch_bounds__(256) void d3q19_mass_boundary(Lattice3<typename T::
 }
 
 // Moments of listed nodes (the values macroscopic() reports there): a node with a boundary
-// slot reads its stash entries, a bulk node its lattice entries, anything else is at rest.
+// slot reads its stash entries, a bulk or side node its lattice entries, anything else is at
+// rest.
 template <class T>
 __global__ __launch_bounds__(256) void d3q19_sample(Lattice3<typename T::R> p, int phase, double fx, double fy, double fz,
                                                     const std::uint32_t* cells, const std::int32_t* slots, std::uint32_t n,
@@ -423,11 +441,12 @@ __global__ __launch_bounds__(256) void d3q19_sample(Lattice3<typename T::R> p, i
             v[i] = static_cast<double>(st[i * p.bstride + static_cast<std::uint32_t>(slot)]);
         }
         moments_of<T>(v, force, rho, ux, uy, uz);
-    } else if (cell < p.cells && p.cls[cell] == 1) {
+    } else if (cell < p.cells && d3::streamed(p.cls[cell])) {
         const auto* f = static_cast<const typename T::S*>(p.f);
+        const std::uint8_t c = p.cls[cell];
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-            v[i] = static_cast<double>(bulk_value<T>(p, f, i, cell, phase));
+            v[i] = static_cast<double>(bulk_value<T>(p, f, i, cell, phase, c));
         }
         moments_of<T>(v, force, rho, ux, uy, uz);
     }
@@ -443,12 +462,13 @@ __global__ __launch_bounds__(256) void d3q19_speed_cells(Lattice3<typename T::R>
                                                          double fz, double* partial) {
     const std::uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
     double speed = 0.0;
-    if (cell < p.cells && p.cls[cell] == 1) {
+    const std::uint8_t c = cell < p.cells ? p.cls[cell] : std::uint8_t(kSolidNode);
+    if (d3::streamed(c)) {
         const auto* f = static_cast<const typename T::S*>(p.f);
         double v[19];
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-            v[i] = static_cast<double>(bulk_value<T>(p, f, i, cell, phase));
+            v[i] = static_cast<double>(bulk_value<T>(p, f, i, cell, phase, c));
         }
         const double force[3] = {fx, fy, fz};
         double rho, ux, uy, uz;
@@ -586,6 +606,14 @@ hipError_t d3q19_step_boundary(const Lattice3<typename T::R>& p, bool local, hip
     return hipGetLastError();
 }
 
+template <class T> hipError_t d3q19_step_side(const Lattice3<typename T::R>& p, bool local, hipStream_t s) {
+    if (local || p.ns == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL((d3q19_side<T>), dim3(blocks_for(p.ns, 256)), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
 hipError_t d3q19_tick(std::uint32_t* clock, hipStream_t s) {
     hipLaunchKernelGGL(d3q19_tick_kernel, dim3(1), dim3(64), 0, s, clock);
     return hipGetLastError();
@@ -604,7 +632,10 @@ template <class T> hipError_t d3q19_step_fused(const Lattice3<typename T::R>& p,
 template <class T>
 hipError_t d3q19_step(const Lattice3<typename T::R>& p, bool local, int block, int cpt, std::uint32_t lds,
                       hipStream_t s) {
-    const hipError_t e = d3q19_step_bulk<T>(p, local, block, cpt, lds, s);
+    hipError_t e = d3q19_step_bulk<T>(p, local, block, cpt, lds, s);
+    if (e == hipSuccess) {
+        e = d3q19_step_side<T>(p, local, s);
+    }
     if (e != hipSuccess) {
         return e;
     }
@@ -701,6 +732,7 @@ hipError_t d3q19_force_sum(const Lattice3<typename T::R>& p, int phase, const st
     template hipError_t d3q19_step<T>(const Lattice3<T::R>&, bool, int, int, std::uint32_t, hipStream_t);    \
     template hipError_t d3q19_step_bulk<T>(const Lattice3<T::R>&, bool, int, int, std::uint32_t, hipStream_t); \
     template hipError_t d3q19_step_boundary<T>(const Lattice3<T::R>&, bool, hipStream_t);                    \
+    template hipError_t d3q19_step_side<T>(const Lattice3<T::R>&, bool, hipStream_t);                        \
     template hipError_t d3q19_step_fused<T>(const Lattice3<T::R>&, bool, hipStream_t);                       \
     template hipError_t d3q19_initialize<T>(const Lattice3<T::R>&, int, const double*, const double*,       \
                                             std::uint64_t, double, hipStream_t);                             \

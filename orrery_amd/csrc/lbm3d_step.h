@@ -355,6 +355,89 @@ __device__ __forceinline__ void bulk_node(const Lattice3<typename T::R>& p, std:
     bulk_finish<T, kLocal>(p, cell, v);
 }
 
+// One of four constants by two flags, formed arithmetically: the compiler turns a select among
+// constants into a table in memory, one dependent load per use.
+__device__ __forceinline__ int pick4(int none, int y, int z, int yz, bool my, bool mz) {
+    const int ky = my ? 1 : 0;
+    const int kz = mz ? 1 : 0;
+    return none + ky * (y - none) + kz * (z - none) + ky * kz * (yz - y - z + none);
+}
+
+// ---- side nodes ---------------------------------------------------------------------------------
+// A side node streams in place like a bulk node. In the A step it reads A[i][x] and writes
+// A[opp(i)][x], its own cell. In the B step pull i comes from A[opp(d)][s], where (s, d) is the
+// source node and direction through the sides (resolve_source: a slip side keeps the row and
+// mirrors the direction, a periodic side wraps), and output opp(i) is written to that same
+// address. A population leaving through a slip side is thereby stored where its mirrored reader
+// loads it in the next A step, a slot that no node streams into across the face; and the B-step
+// reads and writes of a node are again one set of addresses that no other node touches. So side
+// nodes need neither the stash nor source codes.
+
+// Faces a side node lies on: bit 0 y = 0, bit 1 y = ny - 1, bit 2 z = 0, bit 3 z = nz - 1.
+template <class R> __device__ __forceinline__ std::uint32_t side_faces(const Lattice3<R>& p, std::uint32_t cell) {
+    const std::uint32_t z = p.div_plane.div(cell);
+    const std::uint32_t y = p.div_nx.div(cell - z * (p.nx * p.ny));
+    return (y == 0U ? 1U : 0U) | (y + 1U == p.ny ? 2U : 0U) | (z == 0U ? 4U : 0U) | (z + 1U == p.nz ? 8U : 0U);
+}
+
+// Element of the B-step address of pull i of node `cell` lying on `faces` (0: the plain address of
+// a bulk node), counted from g elements before plane 0 (so that the plain addresses of any cell,
+// which reach up to nx ny + nx + 1 cells outside the planes, stay non-negative for g at least
+// that). Selects only: the result is formed without branches, so a run's loads issue together.
+// Every element fits 32 bits: planes hold at most 2^27 cells.
+template <class R>
+__device__ __forceinline__ std::uint32_t side_element(const Lattice3<R>& p, std::uint32_t cell, std::uint32_t faces, int i,
+                                                      std::uint32_t g) {
+    const int cx = L3::cx(i);
+    const int cy = L3::cy(i);
+    const int cz = L3::cz(i);
+    const std::uint32_t ybit = cy == 1 ? 1U : (cy == -1 ? 2U : 0U);
+    const std::uint32_t zbit = cz == 1 ? 4U : (cz == -1 ? 8U : 0U);
+    const bool ycross = (faces & ybit) != 0U;
+    const bool zcross = (faces & zbit) != 0U;
+    const bool yslip = cy != 0 && p.side[cy == 1 ? 0 : 1] == kSlip;
+    const bool zslip = cz != 0 && p.side[cz == 1 ? 2 : 3] == kSlip;
+    const bool my = ycross && yslip;
+    const bool mz = zcross && zslip;
+    // source row and layer offsets: through a slip side the row is kept, through a periodic
+    // side it wraps to the opposite face
+    const int dy = ycross ? (yslip ? 0 : cy * (static_cast<int>(p.ny) - 1)) : -cy;
+    const int dz = zcross ? (zslip ? 0 : cz * (static_cast<int>(p.nz) - 1)) : -cz;
+    const int plane = pick4(L3::opp(i), L3::opp(L3::mirror_y(i)), L3::opp(L3::mirror_z(i)),
+                            L3::opp(L3::mirror_z(L3::mirror_y(i))), my, mz);
+    const int src = static_cast<int>(cell + g) - cx + static_cast<int>(p.nx) * dy + static_cast<int>(p.nx * p.ny) * dz;
+    return static_cast<std::uint32_t>(plane) * p.stride + static_cast<std::uint32_t>(src);
+}
+
+// The B step of one side node (the side kernel's threads, the fused schedule's lanes).
+template <class T> __device__ __forceinline__ void side_node_b(const Lattice3<typename T::R>& p, std::uint32_t cell) {
+    using S = typename T::S;
+    using R = typename T::R;
+    S* __restrict__ f = static_cast<S*>(p.f);
+    const std::uint32_t faces = side_faces(p, cell);
+    R v[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        v[i] = T::load(f[side_element(p, cell, faces, i, 0U)]);
+    }
+    R sponge = 0;
+    if (p.sponge_strength > R(0)) {
+        const std::uint32_t z = p.div_plane.div(cell);
+        const std::uint32_t rest = cell - z * (p.nx * p.ny);
+        const std::uint32_t y = p.div_nx.div(rest);
+        sponge = sponge_sigma3(p, rest - y * p.nx, y, z);
+    }
+    collide3<T>(v, p, sponge);
+    const std::uint32_t bits = T::dithered ? cell_dither(cell, slot_dither(*p.clock)) : 0U;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        f[side_element(p, cell, faces, L3::opp(i), 0U)] = T::pack(v[i], direction_dither(bits, i));
+    }
+}
+
+// A bulk or side node: both update in place; they differ only in the B step's addresses.
+__device__ __forceinline__ bool streamed(std::uint8_t kind) { return kind == kBulkNode || kind == kSideNode; }
+
 // V consecutive stored populations of one plane, as one memory access.
 template <class T, int V> __device__ __forceinline__ void load_run(const typename T::S* src, typename T::R (&out)[V]) {
     typename T::S tmp[V];
@@ -378,10 +461,10 @@ __device__ __forceinline__ void store_run(typename T::S* dst, const typename T::
 }
 
 // The bulk half of a step for the run of V cells starting at c0: each thread takes V
-// consecutive cells and moves each population
-// plane as one V-wide access (19 loads and 19 stores per thread whatever V is). A run whose
-// cells are all bulk nodes stores whole runs; a mixed run stores its bulk cells one by one,
-// because the other cells' addresses belong to nodes the boundary kernel updates.
+// consecutive cells and moves each population plane as one V-wide access (19 loads and 19
+// stores per thread whatever V is). A run whose cells are all updated stores whole runs; a
+// mixed run stores its updated cells one by one, because the other cells' addresses belong to
+// nodes the boundary kernel updates.
 // Element `index` of a population plane: a uniform 64-bit plane base plus a 32-bit byte offset,
 // the addressing of the scalar-base memory instructions (one offset register serves every
 // plane of an A step). Byte offsets stay below 2^32: cells and guards are below 2^28.
@@ -397,15 +480,18 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
     if (c0 >= p.cells) {
         return;
     }
-    // The class array is padded with solid (0) entries up to the plane stride.
+    // The class array is padded with solid (0) entries up to the plane stride. The A step
+    // updates bulk and side nodes (a side node's A step is a bulk node's); the B step bulk nodes
+    // only, side nodes having a kernel of their own then (side_node_b).
     std::uint8_t kind[V];
     __builtin_memcpy(kind, p.cls + c0, sizeof(kind));
     bool any = false;
     bool all = true;
 #pragma unroll
     for (int c = 0; c < V; ++c) {
-        any = any || kind[c] == 1;
-        all = all && kind[c] == 1;
+        const bool mine = kLocal ? streamed(kind[c]) : kind[c] == kBulkNode;
+        any = any || mine;
+        all = all && mine;
     }
     if (!any) {
         return;
@@ -433,9 +519,10 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
             v[c][i] = run[c];
         }
     }
+    const auto updated = [&](int c) { return kLocal ? streamed(kind[c]) : kind[c] == kBulkNode; };
 #pragma unroll
     for (int c = 0; c < V; ++c) {
-        if (kind[c] == 1) {
+        if (updated(c)) {
             R sponge = 0;
             if (p.sponge_strength > R(0)) {
                 const std::uint32_t cell = c0 + c;
@@ -474,7 +561,7 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
     } else {
 #pragma unroll
         for (int c = 0; c < V; ++c) {
-            if (kind[c] == 1) {
+            if (updated(c)) {
                 const std::uint32_t cell = c0 + c;
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
@@ -490,13 +577,16 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
     }
 }
 
-// Direction i after the mirrors a source code asks for, chosen among compile-time constants
-// (i is a constant of an unrolled loop) rather than read from a table at run time.
+// Direction i after the mirrors a source code asks for, and its opposite (i is a constant of an
+// unrolled loop).
 __device__ __forceinline__ std::uint32_t mirrored(int i, std::uint32_t code) {
-    const bool my = (code & kCodeMirrorY) != 0U;
-    const bool mz = (code & kCodeMirrorZ) != 0U;
-    return static_cast<std::uint32_t>(my ? (mz ? L3::mirror_z(L3::mirror_y(i)) : L3::mirror_y(i))
-                                         : (mz ? L3::mirror_z(i) : i));
+    return static_cast<std::uint32_t>(pick4(i, L3::mirror_y(i), L3::mirror_z(i), L3::mirror_z(L3::mirror_y(i)),
+                                            (code & kCodeMirrorY) != 0U, (code & kCodeMirrorZ) != 0U));
+}
+__device__ __forceinline__ std::uint32_t mirrored_opp(int i, std::uint32_t code) {
+    return static_cast<std::uint32_t>(pick4(L3::opp(i), L3::opp(L3::mirror_y(i)), L3::opp(L3::mirror_z(i)),
+                                            L3::opp(L3::mirror_z(L3::mirror_y(i))), (code & kCodeMirrorY) != 0U,
+                                            (code & kCodeMirrorZ) != 0U));
 }
 
 // ---- one node of the step: boundary half -----------------------------------------------------------------
@@ -551,17 +641,19 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
     for (int i = 0; i < 19; ++i) {
         const int kind = static_cast<int>(code[i] & 7U);
         const std::uint32_t payload = code[i] >> kCodeShift;
-        const bool plain = (code[i] & (kCodeMirrorY | kCodeMirrorZ)) == 0U;
         const bool far = kind == kBounce;  // the farther node of a short link (used when q < 1/2)
-        const bool pull = kind == kNode && plain;
+        const bool pull = kind == kNode;   // from a bulk or side node, through the lattice
         if constexpr (kLocal) {
-            // A step: natural pulls are the node's own addresses, direction i or opposite(i)
+            // A step: pulls are the node's own addresses, direction i or opposite(i) (a source
+            // beyond a slip side stored its mirrored output here too)
             fv[i] = (pull || far) ? f[(far ? L3::opp(i) : i) * stride + cell] : S(0);
         } else {
+            // B step: A[opp(d)][s] for the source node s and its direction d after the mirrors
             const std::uint32_t src = far ? static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane)) : payload;
-            fv[i] = (pull || far) ? f[(far ? i : L3::opp(i)) * stride + src] : S(0);
+            const std::uint32_t at = far ? static_cast<std::uint32_t>(i) : mirrored_opp(i, code[i]);
+            fv[i] = (pull || far) ? f[at * stride + src] : S(0);
         }
-        const bool other = (kind == kNode && !plain) || kind == kOutlet || kind == kStash;
+        const bool other = kind == kOutlet || kind == kStash;
         sv[i] = other ? st[mirrored(i, code[i]) * bs + payload] : R(0);
         q[i] = (far && p.bq != nullptr) ? p.bq[i * bs + b] : R(0.5);
     }
@@ -588,7 +680,7 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
         const int kind = static_cast<int>(code[i] & 7U);
         const std::uint32_t payload = code[i] >> kCodeShift;
         if (kind == kNode) {
-            v[i] = (code[i] & (kCodeMirrorY | kCodeMirrorZ)) == 0U ? T::load(fv[i]) : sv[i];
+            v[i] = T::load(fv[i]);
         } else if (kind == kStash) {
             v[i] = sv[i];
         } else if (kind == kBounce) {
@@ -633,20 +725,35 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
         if constexpr (kLocal) {
             f[L3::opp(i) * stride + cell] = packed;
         } else {
+            // The reader of output i and the direction it pulls: x + c_i through the sides
+            // (periodic: wrapped; slip: the row kept and the direction mirrored, a reader that
+            // is a side node, or a bulk node across a periodic side; other readers through a
+            // side read the stash and have no bit in the mask). It loads it from A[d][reader] in
+            // the next A step.
             int tx = static_cast<int>(x) + L3::cx(i);
             int ty = static_cast<int>(y) + L3::cy(i);
             int tz = static_cast<int>(z) + L3::cz(i);
+            bool my = false;
+            bool mz = false;
             bool inside = true;
             if (ty < 0 || ty >= static_cast<int>(p.ny)) {
-                if ((ty < 0 ? p.side[0] : p.side[1]) == kPeriodic) {
+                const int kind = ty < 0 ? p.side[0] : p.side[1];
+                if (kind == kPeriodic) {
                     ty = ty < 0 ? ty + static_cast<int>(p.ny) : ty - static_cast<int>(p.ny);
+                } else if (kind == kSlip) {
+                    ty = static_cast<int>(y);
+                    my = true;
                 } else {
                     inside = false;
                 }
             }
             if (tz < 0 || tz >= static_cast<int>(p.nz)) {
-                if ((tz < 0 ? p.side[2] : p.side[3]) == kPeriodic) {
+                const int kind = tz < 0 ? p.side[2] : p.side[3];
+                if (kind == kPeriodic) {
                     tz = tz < 0 ? tz + static_cast<int>(p.nz) : tz - static_cast<int>(p.nz);
+                } else if (kind == kSlip) {
+                    tz = static_cast<int>(z);
+                    mz = true;
                 } else {
                     inside = false;
                 }
@@ -659,7 +766,8 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
                 }
             }
             if (inside) {
-                f[i * stride + static_cast<std::uint32_t>(tx + nx * ty + plane * tz)] = packed;
+                const int d = pick4(i, L3::mirror_y(i), L3::mirror_z(i), L3::mirror_z(L3::mirror_y(i)), my, mz);
+                f[static_cast<std::uint32_t>(d) * stride + static_cast<std::uint32_t>(tx + nx * ty + plane * tz)] = packed;
             }
         }
     }

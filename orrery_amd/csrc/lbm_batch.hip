@@ -87,6 +87,20 @@ __global__ __launch_bounds__(256) void batch_boundary(const Lattice3<typename T:
     }
 }
 
+// B step of the side nodes of the lattices whose step is a B step (the bulk kernel updates them
+// in A steps).
+template <class T>
+__global__ __launch_bounds__(256) void batch_side(const Lattice3<typename T::R>* __restrict__ lat,
+                                                  const std::uint32_t* __restrict__ start, std::uint32_t n,
+                                                  const std::uint32_t* __restrict__ parity) {
+    const std::uint32_t l = owner(start, n, blockIdx.x);
+    const Lattice3<typename T::R>& p = lat[l];
+    const std::uint32_t s = (blockIdx.x - start[l]) * blockDim.x + threadIdx.x;
+    if (s < p.ns && (parity[l] & 1U) == 0U) {
+        d3::side_node_b<T>(p, p.scell[s]);
+    }
+}
+
 // parity[l] holds lattice l's step count (mod 2^32): bit 0 its A-A parity, the whole word the
 // step clock of the 16-bit formats' rounding dither.
 __global__ void batch_advance(std::uint32_t* parity, std::uint32_t n) {
@@ -230,12 +244,14 @@ struct Group {
     LinkTable* links = nullptr;
     std::uint32_t* bulk_start = nullptr;
     std::uint32_t* bnd_start = nullptr;
+    std::uint32_t* side_start = nullptr;
     std::uint32_t* link_start = nullptr;
     std::uint32_t* parity = nullptr;
     double* partial = nullptr;
     double* force = nullptr; // [n][6]
     std::uint32_t bulk_blocks = 0;
     std::uint32_t bnd_blocks = 0;
+    std::uint32_t side_blocks = 0;
     std::uint32_t link_blocks = 0;
     int block = 256;
     int cpt = 1;
@@ -313,7 +329,7 @@ template <class T> int fill_group(lbm_batch& b, Group& g) {
     const std::size_t n = g.members.size();
     std::vector<lbm::Lattice3<R>> lat(n);
     std::vector<lbm::LinkTable> links(n);
-    std::vector<std::uint32_t> bulk(n + 1, 0), bnd(n + 1, 0), lnk(n + 1, 0), parity(n, 0);
+    std::vector<std::uint32_t> bulk(n + 1, 0), bnd(n + 1, 0), side(n + 1, 0), lnk(n + 1, 0), parity(n, 0);
     const std::uint32_t per = static_cast<std::uint32_t>(g.block * g.cpt);
     for (std::size_t k = 0; k < n; ++k) {
         lbm3d* h = b.entries[static_cast<std::size_t>(g.members[k])].h;
@@ -322,17 +338,20 @@ template <class T> int fill_group(lbm_batch& b, Group& g) {
         links[k] = lbm::LinkTable{v.cell, v.dir, v.q, v.slot, v.count};
         bulk[k + 1] = bulk[k] + lbm::ceil_div(lat[k].cells, per);
         bnd[k + 1] = bnd[k] + std::max<std::uint32_t>(1, lbm::ceil_div(lat[k].nb, 256));
+        side[k + 1] = side[k] + std::max<std::uint32_t>(1, lbm::ceil_div(lat[k].ns, 256));
         lnk[k + 1] = lnk[k] + std::max<std::uint32_t>(1, lbm::ceil_div(v.count, 256));
         parity[k] = static_cast<std::uint32_t>(lbm3d_steps(h)); // the step clock too (mod 2^32)
     }
     g.bulk_blocks = bulk[n];
     g.bnd_blocks = bnd[n];
+    g.side_blocks = side[n];
     g.link_blocks = lnk[n];
     int rc = LBM_OK;
     // (the descriptors' clocks point into g.parity once it is allocated, below)
     if ((rc = dalloc(b, reinterpret_cast<char**>(&g.lat), n * sizeof(lbm::Lattice3<R>))) ||
         (rc = dalloc(b, &g.links, n * sizeof(lbm::LinkTable))) || (rc = dalloc(b, &g.bulk_start, (n + 1) * 4)) ||
-        (rc = dalloc(b, &g.bnd_start, (n + 1) * 4)) || (rc = dalloc(b, &g.link_start, (n + 1) * 4)) ||
+        (rc = dalloc(b, &g.bnd_start, (n + 1) * 4)) || (rc = dalloc(b, &g.side_start, (n + 1) * 4)) ||
+        (rc = dalloc(b, &g.link_start, (n + 1) * 4)) ||
         (rc = dalloc(b, &g.parity, n * 4)) || (rc = dalloc(b, &g.partial, static_cast<std::size_t>(g.link_blocks) * 48)) ||
         (rc = dalloc(b, &g.force, n * 48))) {
         return rc;
@@ -347,6 +366,7 @@ template <class T> int fill_group(lbm_batch& b, Group& g) {
     if (e == hipSuccess) e = up(g.links, links.data(), n * sizeof(lbm::LinkTable));
     if (e == hipSuccess) e = up(g.bulk_start, bulk.data(), (n + 1) * 4);
     if (e == hipSuccess) e = up(g.bnd_start, bnd.data(), (n + 1) * 4);
+    if (e == hipSuccess) e = up(g.side_start, side.data(), (n + 1) * 4);
     if (e == hipSuccess) e = up(g.link_start, lnk.data(), (n + 1) * 4);
     if (e == hipSuccess) e = up(g.parity, parity.data(), n * 4);
     if (e == hipSuccess) e = hipStreamSynchronize(b.stream);
@@ -371,6 +391,7 @@ template <class T> void enqueue_step(Group& g, hipStream_t s) {
                                g.parity);
             break;
     }
+    hipLaunchKernelGGL((lbm::batch_side<T>), dim3(g.side_blocks), dim3(256), 0, s, lat, g.side_start, n, g.parity);
     hipLaunchKernelGGL((lbm::batch_boundary<T>), dim3(g.bnd_blocks), dim3(256), 0, s, lat, g.bnd_start, n, g.parity);
     hipLaunchKernelGGL(lbm::batch_advance, dim3(1), dim3(256), 0, s, g.parity, n);
 }
@@ -724,7 +745,8 @@ int lbm_batch_run_timed(lbm_batch* b, uint64_t steps, lbm_batch_timing* timing) 
         }
         b->pinned_doubles = need;
     }
-    // Marks: per step and group, before the bulk, before the boundary and after it; per chunk,
+    // Marks: per step and group, before the bulk kernel, after the side kernel (before the
+    // boundary kernel) and after the boundary kernel; per chunk,
     // before and after the force kernels; one at the very end.
     const std::size_t groups = b->groups.size();
     const std::size_t per_chunk = b->sample_every * groups * 3U + 2U;
@@ -760,6 +782,8 @@ int lbm_batch_run_timed(lbm_batch* b, uint64_t steps, lbm_batch_timing* timing) 
                                                b->stream, lat, g.bulk_start, count, g.parity);
                             break;
                     }
+                    hipLaunchKernelGGL((lbm::batch_side<T>), dim3(g.side_blocks), dim3(256), 0, b->stream, lat,
+                                       g.side_start, count, g.parity);
                     (void)hipEventRecord(ev[m++], b->stream);
                     hipLaunchKernelGGL((lbm::batch_boundary<T>), dim3(g.bnd_blocks), dim3(256), 0, b->stream, lat,
                                        g.bnd_start, count, g.parity);
@@ -821,11 +845,11 @@ int lbm_batch_run_timed(lbm_batch* b, uint64_t steps, lbm_batch_timing* timing) 
         for (const lbm::Group& g : b->groups) {
             for (const int member : g.members) {
                 std::size_t bulk = 0;
-                std::size_t boundary = 0;
-                (void)lbm3d_counts(b->entries[static_cast<std::size_t>(member)].h, &bulk, &boundary, nullptr,
-                                   nullptr);
-                timing->bulk_nodes[g.storage] += bulk;
-                timing->boundary_nodes[g.storage] += boundary;
+                std::size_t side = 0;
+                std::size_t listed = 0;
+                (void)lbm3d_stream_counts(b->entries[static_cast<std::size_t>(member)].h, &bulk, &side, &listed);
+                timing->bulk_nodes[g.storage] += bulk + side;
+                timing->boundary_nodes[g.storage] += listed;
             }
         }
     }

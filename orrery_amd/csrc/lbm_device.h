@@ -131,11 +131,21 @@ struct StoreF64 {
     static __device__ __forceinline__ S from_double(double v) { return v; }
 };
 
+// ---- node classes ----------------------------------------------------------------------------
+// 0 solid; 1 bulk (all 18 neighbours in the domain and fluid); 2 boundary (listed: faces and
+// neighbours of solids, updated by the boundary kernel through source codes and the stash);
+// 3 side (a fluid node on a y or z face, away from the x faces and from solids, whose every
+// source lies in the lattice after the side's mirror or wrap: streamed by the bulk kernel). The
+// reference's classification (classify_nodes) counts side nodes as boundary nodes.
+enum NodeClass : std::uint8_t { kSolidNode = 0, kBulkNode = 1, kListedNode = 2, kSideNode = 3 };
+
 // ---- boundary-node source codes -------------------------------------------------------------
 // One 32-bit word per (boundary node, direction), built on the host from resolve_d3q19:
 //   bits 0..2 kind (SourceKind), bit 3 mirror_y applied, bit 4 mirror_z applied,
-//   bits 5..31 payload: kNode unmirrored -> source cell; kNode mirrored / kOutlet / kStash ->
-//   source boundary slot; kWall -> side (0..3).
+//   bits 5..31 payload: kNode -> source cell (a bulk or side node, read from the lattice);
+//   kStash / kOutlet -> source boundary slot (read from the stash); kWall -> side (0..3).
+// The D2Q9 tables (resolve_d2q9) have no side nodes: there a mirrored kNode carries the source
+// boundary slot and is read from the stash.
 constexpr std::uint32_t kCodeMirrorY = 8U;
 constexpr std::uint32_t kCodeMirrorZ = 16U;
 constexpr int kCodeShift = 5;
@@ -149,7 +159,7 @@ __host__ __device__ constexpr std::uint32_t make_code(int kind, bool my, bool mz
 // ---- lattice descriptor (kernel argument) ---------------------------------------------------
 template <typename R> struct Lattice3 {
     void* f;                  // S[19][stride]
-    const std::uint8_t* cls;  // node class per cell: 0 solid, 1 bulk, 2 boundary
+    const std::uint8_t* cls;  // node class per cell (NodeClass)
     const std::uint32_t* bcell;  // boundary node -> cell [nb]
     const std::uint32_t* bcode;  // [19][bstride]
     const std::uint32_t* bwrite; // [bstride] bit i: the lattice copy of output i has a reader
@@ -157,8 +167,9 @@ template <typename R> struct Lattice3 {
     R* stash;                    // [2][19][bstride]
     R* hist;                     // [5][bstride] convective-outlet history (outlet nodes)
     const std::uint32_t* tile_slot;  // [cells / 64 + 1] boundary nodes before each 64-cell run
+    const std::uint32_t* scell;      // side node -> cell [ns] (cell order): the side kernel's list
     const std::uint32_t* clock;      // steps since initialisation (the 16-bit formats' dither epoch)
-    std::uint32_t nx, ny, nz, cells, stride, nb, bstride;
+    std::uint32_t nx, ny, nz, cells, stride, nb, bstride, ns;
     FastDiv div_nx, div_plane;
     int x_mode;       // 0 periodic, 1 inflow/outflow
     int convective;   // outlet blends with history

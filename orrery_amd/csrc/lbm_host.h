@@ -36,12 +36,15 @@ struct Link {
 
 // Everything the device lattice needs from the mask.
 struct Tables {
-    std::vector<std::uint8_t> cls;       // 0 solid, 1 bulk, 2 boundary
+    // 0 solid, 1 bulk, 2 boundary (listed), 3 side (D3Q19: streamed in place by the bulk kernel;
+    // the reference's classification counts them as boundary nodes)
+    std::vector<std::uint8_t> cls;
     std::vector<std::uint32_t> bcell;    // boundary node -> cell (cell order)
+    std::vector<std::uint32_t> scell;    // D3Q19: side node -> cell (cell order)
     std::vector<std::uint32_t> bcode;    // [Q][bstride] source codes
     // D3Q19: per boundary node, bit i set when its output i must also be stored in the lattice
-    // (the node it streams to reads the lattice there: a bulk node, or the far read of a short
-    // bounce-back link); boundary readers of boundary sources read the stash instead.
+    // (the node it streams to reads the lattice there: a bulk or side node, or the far read of a
+    // short bounce-back link); boundary readers of boundary sources read the stash instead.
     std::vector<std::uint32_t> bwrite;
     std::vector<double> bq;              // [Q][bstride] wall fractions (interpolated wall only)
     std::vector<Link> links;             // boundary_links order

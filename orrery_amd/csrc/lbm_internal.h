@@ -59,6 +59,7 @@ struct lbm3d {
     double* lq = nullptr;
     std::uint32_t* lslot = nullptr;
     std::uint32_t* tile_slot = nullptr;
+    std::uint32_t* scell = nullptr;
     double* partial = nullptr;
     double* result = nullptr;
     // Steps since initialisation, on the device (after the reduction results): the 16-bit
@@ -123,6 +124,7 @@ inline std::size_t lattice_bytes(std::size_t cells, std::uint32_t stride, std::s
     total += stride;                                                    // node classes (u8, padded)
     total += t.bcell.size() * 4U;                                       // boundary list
     total += t.tile_slot.size() * 4U;                                   // boundary slot per 64 cells
+    total += t.scell.size() * 4U;                                       // side list
     total += 19U * static_cast<std::size_t>(t.bstride) * 4U;            // source codes
     total += static_cast<std::size_t>(t.bstride) * 4U;                  // lattice write masks
     if (interpolated) {
@@ -160,6 +162,7 @@ template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
     p.stash = static_cast<R*>(h.stash);
     p.hist = static_cast<R*>(h.hist);
     p.tile_slot = h.tile_slot;
+    p.scell = h.scell;
     p.clock = h.clock;
     p.nx = static_cast<std::uint32_t>(c.nx);
     p.ny = static_cast<std::uint32_t>(c.ny);
@@ -167,6 +170,7 @@ template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
     p.cells = static_cast<std::uint32_t>(h.cells);
     p.stride = h.stride;
     p.nb = static_cast<std::uint32_t>(h.tables.bcell.size());
+    p.ns = static_cast<std::uint32_t>(h.tables.scell.size());
     p.bstride = h.tables.bstride;
     p.div_nx = lbm::FastDiv(p.nx);
     p.div_plane = lbm::FastDiv(p.nx * p.ny);

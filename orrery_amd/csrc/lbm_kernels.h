@@ -10,15 +10,18 @@
 
 namespace lbm {
 
-// One lattice step: bulk kernel over all cells, boundary kernel over the boundary list.
-// `local` is the A step (odd step count), otherwise the B step.
+// One lattice step: bulk kernel over all cells, in B steps the side kernel over the side list,
+// boundary kernel over the boundary list. `local` is the A step (odd step count), otherwise the
+// B step.
 template <class T>
 hipError_t d3q19_step(const Lattice3<typename T::R>& p, bool local, int block, int cpt, std::uint32_t lds,
                       hipStream_t s);
-// The two halves of a step, for per-kernel timing.
+// The parts of a step, for per-kernel timing and the two-stream schedule (the side kernel
+// launches nothing in A steps).
 template <class T>
 hipError_t d3q19_step_bulk(const Lattice3<typename T::R>& p, bool local, int block, int cpt, std::uint32_t lds,
                            hipStream_t s);
+template <class T> hipError_t d3q19_step_side(const Lattice3<typename T::R>& p, bool local, hipStream_t s);
 template <class T>
 hipError_t d3q19_step_boundary(const Lattice3<typename T::R>& p, bool local, hipStream_t s);
 // Advances a lattice's step clock by one (the end of a step of a dithered 16-bit format).

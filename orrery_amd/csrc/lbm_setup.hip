@@ -319,6 +319,17 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
         offset[i] = -static_cast<std::ptrdiff_t>(D3Q19::cx(i)) - static_cast<std::ptrdiff_t>(nx) * D3Q19::cy(i) -
                     static_cast<std::ptrdiff_t>(plane) * D3Q19::cz(i);
     }
+    const int sides[4] = {c.y_low.kind, c.y_high.kind, c.z_low.kind, c.z_high.kind};
+    const int inx = static_cast<int>(nx);
+    const int iny = static_cast<int>(ny);
+    const int inz = static_cast<int>(nz);
+    const auto solid_at = [&](int a, int b, int d) {
+        return solid[static_cast<std::size_t>(a) + nx * (static_cast<std::size_t>(b) + ny * static_cast<std::size_t>(d))] != 0;
+    };
+    const auto cell_of = [&](int a, int b, int d) {
+        return static_cast<std::uint32_t>(static_cast<std::size_t>(a) +
+                                          nx * (static_cast<std::size_t>(b) + ny * static_cast<std::size_t>(d)));
+    };
     std::size_t nb = 0;
     for (std::size_t z = 0; z < nz; ++z) {
         for (std::size_t y = 0; y < ny; ++y) {
@@ -332,7 +343,20 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
                 }
                 if (face_row || x == 0 || x + 1 == nx) {
                     t.cls[here] = 2;
-                    ++nb;
+                    // A side node: on a y or z face, not on an x face, and every source in the
+                    // lattice after the sides' mirrors and wraps (no wall, solid, inlet or outlet
+                    // source). It streams in place through the bulk kernel.
+                    bool side = face_row && x > 0 && x + 1 < nx;
+                    for (int i = 1; i < 19 && side; ++i) {
+                        const Resolved r = resolve_d3q19(static_cast<int>(x), static_cast<int>(y), static_cast<int>(z), i,
+                                                         inx, iny, inz, c.x_boundary, sides, solid_at);
+                        side = r.kind == kNode;
+                    }
+                    if (side) {
+                        t.cls[here] = 3;
+                    } else {
+                        ++nb;
+                    }
                     continue;
                 }
                 for (int i = 1; i < 19; ++i) {
@@ -356,6 +380,8 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
         if (t.cls[cell] == 2) {
             slot_of[cell] = static_cast<std::uint32_t>(t.bcell.size());
             t.bcell.push_back(static_cast<std::uint32_t>(cell));
+        } else if (t.cls[cell] == 3) {
+            t.scell.push_back(static_cast<std::uint32_t>(cell));
         }
     }
     if ((cells & 63U) == 0) {
@@ -366,17 +392,6 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
     if (c.solid_wall == LBM_SOLID_INTERPOLATED) {
         t.bq.assign(static_cast<std::size_t>(19) * t.bstride, 0.5);
     }
-    const int sides[4] = {c.y_low.kind, c.y_high.kind, c.z_low.kind, c.z_high.kind};
-    const int inx = static_cast<int>(nx);
-    const int iny = static_cast<int>(ny);
-    const int inz = static_cast<int>(nz);
-    const auto solid_at = [&](int a, int b, int d) {
-        return solid[static_cast<std::size_t>(a) + nx * (static_cast<std::size_t>(b) + ny * static_cast<std::size_t>(d))] != 0;
-    };
-    const auto cell_of = [&](int a, int b, int d) {
-        return static_cast<std::uint32_t>(static_cast<std::size_t>(a) +
-                                          nx * (static_cast<std::size_t>(b) + ny * static_cast<std::size_t>(d)));
-    };
     for (std::size_t b = 0; b < nb; ++b) {
         const std::size_t cell = t.bcell[b];
         const int z = static_cast<int>(cell / plane);
@@ -390,13 +405,12 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
             switch (r.kind) {
                 case kNode: {
                     const std::uint32_t src = cell_of(r.sx, r.sy, r.sz);
-                    if (r.mirror != 0) {
-                        code = make_code(kNode, my, mz, slot_of[src]);
-                    } else if (t.cls[src] == 2) {
-                        // The source's output i is also in its stash slot, densely.
-                        code = make_code(kStash, false, false, slot_of[src]);
+                    if (t.cls[src] == 2) {
+                        // The source's output is also in its stash slot, densely.
+                        code = make_code(kStash, my, mz, slot_of[src]);
                     } else {
-                        code = make_code(kNode, false, false, src);
+                        // A bulk or side node: through the lattice (mirrored only for side nodes).
+                        code = make_code(kNode, my, mz, src);
                     }
                     break;
                 }
@@ -426,10 +440,12 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
         }
     }
     // Which outputs of a boundary node need their lattice copy: output i of node w streams to
-    // n = w + c_i (through periodic sides); n reads it from the lattice when n is a bulk node,
-    // or when n is a boundary node whose link opposite(i) is a bounce-back (the far read of a
-    // short link, also made by the force). Every other reader is a boundary node reading w's
-    // stash (kStash), and targets outside the domain or solid have no reader.
+    // n = w + c_i through the sides (periodic: wrapped; slip: the row kept, the direction
+    // mirrored); n reads it from the lattice when n is a bulk or side node, or when n is a
+    // boundary node, reached without a mirror, whose link opposite(i) is a bounce-back (the far
+    // read of a short link, also made by the force). Every other reader is a boundary node
+    // reading w's stash (kStash), and targets through walls, inlet, outlet or into solids have
+    // no reader.
     t.bwrite.assign(t.bstride, 0U);
     for (std::size_t b = 0; b < nb; ++b) {
         const std::size_t cell = t.bcell[b];
@@ -442,9 +458,13 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
             int ty = y + D3Q19::cy(i);
             int tz = z + D3Q19::cz(i);
             bool inside = true;
+            bool mirrored = false;
             if (ty < 0 || ty >= iny) {
                 if (sides[ty < 0 ? 0 : 1] == LBM_WALL_PERIODIC) {
                     ty = ty < 0 ? ty + iny : ty - iny;
+                } else if (sides[ty < 0 ? 0 : 1] == LBM_WALL_SLIP) {
+                    ty = y;
+                    mirrored = true;
                 } else {
                     inside = false;
                 }
@@ -452,6 +472,9 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
             if (tz < 0 || tz >= inz) {
                 if (sides[tz < 0 ? 2 : 3] == LBM_WALL_PERIODIC) {
                     tz = tz < 0 ? tz + inz : tz - inz;
+                } else if (sides[tz < 0 ? 2 : 3] == LBM_WALL_SLIP) {
+                    tz = z;
+                    mirrored = true;
                 } else {
                     inside = false;
                 }
@@ -467,8 +490,8 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
                 continue;
             }
             const std::uint32_t n = cell_of(tx, ty, tz);
-            bool needed = t.cls[n] == 1;
-            if (t.cls[n] == 2) {
+            bool needed = t.cls[n] == 1 || t.cls[n] == 3;
+            if (t.cls[n] == 2 && !mirrored) {
                 const std::uint32_t far = t.bcode[static_cast<std::size_t>(D3Q19::opp(i)) * t.bstride + slot_of[n]];
                 needed = static_cast<int>(far & 7U) == kBounce;
             }
@@ -646,7 +669,8 @@ int lbm3d_classify(const lbm3d_config* config, const uint8_t* solid, uint8_t* cl
         return rc;
     }
     const lbm::host::Tables t = lbm::host::tables3(c, solid, nullptr);
-    std::copy(t.cls.begin(), t.cls.end(), classes);
+    // the reference's classes: side nodes are boundary nodes
+    std::transform(t.cls.begin(), t.cls.end(), classes, [](std::uint8_t k) { return k == 3 ? std::uint8_t{2} : k; });
     return LBM_OK;
 }
 

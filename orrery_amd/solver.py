@@ -244,6 +244,14 @@ class Lbm3DSolver:
         _check(self._lib, self._lib.lbm3d_counts(self._h, *[ctypes.byref(x) for x in v]))
         return NodeCounts(*(x.value for x in v))
 
+    def stream_counts(self) -> tuple:
+        """lbm3d_stream_counts: (bulk, side, listed) fluid nodes: the bulk kernel streams the bulk
+        nodes, and in A steps the side nodes (boundary nodes on slip or periodic y and z faces),
+        which a side kernel updates in B steps; the boundary kernel the listed nodes."""
+        v = [ctypes.c_size_t() for _ in range(3)]
+        _check(self._lib, self._lib.lbm3d_stream_counts(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     def set_schedule(self, schedule: str) -> None:
         """"serial" (bulk then boundary kernel, the default), "fused" (one kernel per step) or
         "streams" (the two kernels concurrently on two streams); all give identical results."""

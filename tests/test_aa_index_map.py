@@ -14,7 +14,8 @@ steps of a lattice whose interior nodes are bulk nodes: every node reads exactly
 it writes (the in-place property), no address is written twice in a step, a bulk node's pull i
 is the label its upstream bulk node x - c_i wrote one step before (the streaming), and the
 readback map finds every label. A planted mistake in the map (the B step's store without its
-shift) breaks the streaming check.
+shift) breaks the streaming check. The second half does the same for side nodes (face nodes
+streamed in place through slip and periodic sides) next to the listed boundary nodes.
 """
 import numpy as np
 import pytest
@@ -112,3 +113,167 @@ def test_b_step_sources_stay_inside_the_guards():
             loads, stores = lat.addresses(x, 0)
             for _, e in loads + stores:
                 assert -lat.guard <= e < lat.cells + lat.guard
+
+
+# ---- side nodes and the boundary nodes around them ------------------------------------------------
+# A lattice with slip or periodic y and z sides and an inflow/outflow x: the rows on the y and z
+# faces (away from the x faces) hold side nodes, streamed in place by the bulk kernel with the
+# B-step address of pull i at plane opp(d), element s, (s, d) the source node and direction
+# through the sides; every other face node is a listed boundary node, which reads the outputs
+# of listed sources from their stash, those of bulk and side sources from the lattice (A step:
+# its own cell, plane i; B step: plane opp(d) at s), and stores an output in the lattice when
+# its reader is a bulk or side node (A step: its own cell, plane opp(k); B step: the reader's
+# A-step address, plane d' at r, (r, d') the reader and its pull through the sides).
+MY = [0, 1, 2, 4, 3, 5, 6, 9, 10, 7, 8, 11, 12, 13, 14, 18, 17, 16, 15]
+MZ = [0, 1, 2, 3, 4, 6, 5, 7, 8, 9, 10, 13, 14, 11, 12, 17, 18, 15, 16]
+SLIP, PERIODIC = "slip", "periodic"
+
+
+class SideLattice:
+    def __init__(self, nx, ny, nz, sides, mirror_side_loads=True):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.cells = nx * ny * nz
+        self.sides = sides  # y low, y high, z low, z high
+        self.mirror_side_loads = mirror_side_loads
+        self.mem = {}
+        self.stash = {}
+        self.cls = {}
+        for x in range(self.cells):
+            xx, yy, zz = self.xyz(x)
+            face = yy in (0, ny - 1) or zz in (0, nz - 1)
+            if xx in (0, nx - 1):
+                self.cls[x] = 2
+            elif face:
+                self.cls[x] = 3 if all(self.resolve(x, i)[0] == "node" for i in range(19)) else 2
+            else:
+                self.cls[x] = 1
+
+    def xyz(self, x):
+        return x % self.nx, (x // self.nx) % self.ny, x // (self.nx * self.ny)
+
+    def at(self, a, b, c):
+        return a + self.nx * (b + self.ny * c)
+
+    def resolve(self, x, i):
+        """(kind, source node, source direction) of pull i: y sides, then z, then x."""
+        a, b, c = self.xyz(x)
+        sa, sb, sc, d = a - CX[i], b - CY[i], c - CZ[i], i
+        if not 0 <= sb < self.ny:
+            if self.sides[0 if sb < 0 else 1] == PERIODIC:
+                sb %= self.ny
+            else:
+                sb, d = b, MY[d]
+        if not 0 <= sc < self.nz:
+            if self.sides[2 if sc < 0 else 3] == PERIODIC:
+                sc %= self.nz
+            else:
+                sc, d = c, MZ[d]
+        if not 0 <= sa < self.nx:
+            return ("inlet" if sa < 0 else "outlet", None, None)
+        return ("node", self.at(sa, sb, sc), d)
+
+    def reader(self, x, k):
+        """(reader node, its pull direction) of output k of node x, or None."""
+        a, b, c = self.xyz(x)
+        ta, tb, tc, d = a + CX[k], b + CY[k], c + CZ[k], k
+        if not 0 <= tb < self.ny:
+            if self.sides[0 if tb < 0 else 1] == PERIODIC:
+                tb %= self.ny
+            else:
+                tb, d = b, MY[d]
+        if not 0 <= tc < self.nz:
+            if self.sides[2 if tc < 0 else 3] == PERIODIC:
+                tc %= self.nz
+            else:
+                tc, d = c, MZ[d]
+        if not 0 <= ta < self.nx:
+            return None
+        return self.at(ta, tb, tc), d
+
+    def b_address(self, x, i):
+        """B-step load address of pull i of a bulk or side node (the store of output opp(i))."""
+        kind, s, d = self.resolve(x, i)
+        assert kind == "node"
+        if self.cls[x] == 3 and not self.mirror_side_loads:
+            return (OPP[i], x - CX[i] - self.nx * CY[i] - self.nx * self.ny * CZ[i])
+        return (OPP[d], s)
+
+    def readback(self, x, i, n):
+        return (OPP[i], x) if n % 2 == 0 else self.b_address(x, OPP[i])
+
+
+def run_sides(lat, steps):
+    streamed = [x for x in range(lat.cells) if lat.cls[x] in (1, 3)]
+    listed = [x for x in range(lat.cells) if lat.cls[x] == 2]
+    for x in range(lat.cells):
+        for i in range(19):
+            lat.mem[(OPP[i], x)] = (x, i, 0)
+            if lat.cls[x] == 2:
+                lat.stash[(x, i)] = (x, i, 0)
+    failures = []
+    for n in range(steps):
+        local = n % 2 == 1
+        written, pulled = {}, {}
+        for x in streamed:
+            if local:
+                loads = [(i, x) for i in range(19)]
+                stores = [(OPP[i], x) for i in range(19)]
+            else:
+                loads = [lat.b_address(x, i) for i in range(19)]
+                stores = [lat.b_address(x, OPP[i]) for i in range(19)]
+            if sorted(loads) != sorted(stores):
+                failures.append(("not in place", x, n))
+            for _, e in loads:
+                if not 0 <= e < lat.cells:
+                    failures.append(("outside the planes", x, n))
+            for i, a in enumerate(loads):
+                pulled[(x, i)] = lat.mem.get(a)
+            for k, a in enumerate(stores):
+                if a in written:
+                    failures.append(("written twice", a, n))
+                written[a] = (x, k, n + 1)
+        new_stash = {}
+        for x in listed:
+            for i in range(19):
+                kind, s, d = lat.resolve(x, i)
+                if kind != "node":
+                    continue
+                if lat.cls[s] == 2:
+                    pulled[(x, i)] = lat.stash[(s, d)]
+                else:
+                    pulled[(x, i)] = lat.mem.get((i, x) if local else (OPP[d], s))
+            for k in range(19):
+                new_stash[(x, k)] = (x, k, n + 1)
+                r = lat.reader(x, k)
+                if r is None or lat.cls[r[0]] == 2:
+                    continue
+                a = (OPP[k], x) if local else (r[1], r[0])
+                if a in written:
+                    failures.append(("written twice", a, n))
+                written[a] = (x, k, n + 1)
+        for x in streamed + listed:
+            for i in range(19):
+                kind, s, d = lat.resolve(x, i)
+                if kind == "node" and pulled[(x, i)] != (s, d, n):
+                    failures.append(("stream", x, i, n, pulled[(x, i)], (s, d, n)))
+        lat.mem.update(written)
+        lat.stash.update(new_stash)
+        for x in streamed:
+            for i in range(19):
+                if lat.mem.get(lat.readback(x, i, n + 1)) != (x, i, n + 1):
+                    failures.append(("readback", x, i, n + 1))
+    return failures
+
+
+@pytest.mark.parametrize("sides", [(SLIP, SLIP, SLIP, SLIP), (PERIODIC, PERIODIC, SLIP, SLIP),
+                                   (SLIP, SLIP, PERIODIC, PERIODIC), (PERIODIC, PERIODIC, PERIODIC, PERIODIC)])
+@pytest.mark.parametrize("dims", [(6, 5, 4), (5, 3, 6)])
+def test_side_nodes_stream_in_place_through_slip_and_periodic_sides(dims, sides):
+    lat = SideLattice(*dims, sides)
+    assert any(c == 3 for c in lat.cls.values())
+    assert run_sides(lat, 6) == []
+
+
+def test_side_nodes_without_their_mirrored_addresses_are_caught():
+    failures = run_sides(SideLattice(6, 5, 4, (SLIP,) * 4, mirror_side_loads=False), 3)
+    assert any(f[0] == "stream" for f in failures)

@@ -149,6 +149,50 @@ def test_boundary_sources_and_lattice_stores_f64_match_the_oracle(case):
         assert out["force_x"] < 1e-10
 
 
+@pytest.mark.parametrize("sides", ["slip_slip", "slip_periodic", "periodic_slip"])
+def test_side_nodes_next_to_boundary_nodes_f64_match_the_oracle(sides):
+    """Side nodes (the y and z face nodes streamed in place through slip mirrors and periodic
+    wraps, by the bulk kernel in A steps and the side kernel in B steps) next to listed boundary
+    nodes: bodies cut by the faces, so that boundary nodes pull from side nodes through the
+    lattice (mirrored or wrapped) and store their outputs where side-node readers load them.
+    Odd and even step counts, the FP64 storage against the oracle to rounding."""
+    y, z = {"slip_slip": (WallKind.slip, WallKind.slip), "slip_periodic": (WallKind.slip, WallKind.periodic),
+            "periodic_slip": (WallKind.periodic, WallKind.slip)}[sides]
+    c, _ = sphere_channel(34, 16, 14, 6.0, 2.0, 0.04, 30.0)
+    c.y_low, c.y_high, c.z_low, c.z_high = wall(y), wall(y), wall(z), wall(z)
+    c.smagorinsky = Smagorinsky(True, 0.12)
+    c.solid_wall = SolidWall.interpolated
+    c.sponge = Sponge(0.1, 3, 5, 2)
+    # one sphere through the y = 0 face, one through the z = nz - 1 face
+    mask = _two_spheres(34, 16, 14, (9.3, 0.6, 6.2), 2.6, (22.4, 8.3, 13.2), 2.4)
+    for steps in (1, 2, 53):
+        out, cpu, gpu = compare(c, mask, steps, "f64", f"sides {sides} {steps}")
+        bulk, side, listed = gpu.stream_counts()
+        counts = gpu.counts()
+        assert side > 0 and bulk == counts.bulk_nodes and side + listed == counts.boundary_nodes
+        assert out["populations"] < 1e-13
+        assert out["velocity"] < 1e-12
+        assert out["force_x"] < 1e-10
+        # the same populations summed in another order (device block partials, oracle rows):
+        # 1.4e5 terms of FP64
+        assert out["mass"] < 1e-11
+
+
+def test_side_nodes_of_a_slip_channel():
+    """In a channel with slip sides, every face node away from the inlet and outlet faces and
+    from the body is a side node."""
+    c, mask = SPHERE
+    gpu = Lbm3DSolver(c, mask, "f32")
+    bulk, side, listed = gpu.stream_counts()
+    nx, ny, nz = c.nx, c.ny, c.nz
+    face_rows = 2 * nz + 2 * (ny - 2)
+    assert side == face_rows * (nx - 2)  # the sphere does not reach the faces
+    # the inlet and outlet faces and the fluid neighbours of the sphere are listed
+    assert listed > 2 * ny * nz
+    assert bulk + side + listed == c.cells - gpu.counts().solid_nodes
+    gpu.close()
+
+
 def test_perturbed_start_is_identical_on_device_and_host():
     c, mask = sphere_channel(32, 16, 16, 6.0, 2.0, 0.05, 50.0)
     cpu = Oracle3D(c, mask)
