@@ -214,10 +214,12 @@ int lbm3d_step_bytes(const lbm3d* h, double* bytes);
  * reference classifies them (classify_nodes, lbm3d_internal.h:219-248). */
 int lbm3d_counts(const lbm3d* h, size_t* bulk, size_t* boundary, size_t* solid, size_t* links);
 /* No counterpart in the reference (its kernels do not split the boundary nodes): how this
- * library's step kernels divide the fluid nodes. `bulk` and `side` nodes are streamed in place by
- * the bulk kernel (side nodes: the boundary nodes on slip or periodic y and z faces, away from the
- * x faces and from solids); `listed` nodes by the boundary kernel, through source codes and the
- * stash. bulk + side + listed is the number of fluid nodes. */
+ * library's step kernels divide the fluid nodes. `bulk` nodes (with the nodes of the velocity
+ * inlet face off the y and z faces and away from solids) are streamed in place by the bulk
+ * kernel; `side` nodes (the boundary nodes on slip or periodic y and z faces, away from the x
+ * faces and from solids) by the bulk kernel in A steps and the side kernel in B steps; `listed`
+ * nodes by the boundary kernel, through source codes and the stash. bulk + side + listed is the
+ * number of fluid nodes. */
 int lbm3d_stream_counts(const lbm3d* h, size_t* bulk, size_t* side, size_t* listed);
 /* Advances `count` steps and times them with HIP events on the handle's stream (the
  * reference's bench protocol, tests/bench/aero/bench_lbm_gpu.cpp:54-64). With per_kernel = 0,

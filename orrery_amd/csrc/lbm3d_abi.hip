@@ -863,12 +863,12 @@ int lbm3d_counts(const lbm3d* h, size_t* bulk, size_t* boundary, size_t* solid, 
     if (h == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null handle");
     }
-    std::size_t n[4] = {0, 0, 0, 0};
+    std::size_t n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (const std::uint8_t c : h->tables.cls) {
-        ++n[c & 3U];
+        ++n[c & 7U];
     }
     if (bulk != nullptr) *bulk = n[1];
-    if (boundary != nullptr) *boundary = n[2] + n[3];
+    if (boundary != nullptr) *boundary = n[2] + n[3] + n[4];
     if (solid != nullptr) *solid = n[0];
     if (links != nullptr) *links = h->tables.links.size();
     return ok();
@@ -878,11 +878,12 @@ int lbm3d_stream_counts(const lbm3d* h, size_t* bulk, size_t* side, size_t* list
     if (h == nullptr) {
         return fail(LBM_EINVAL, "lbm3d: null handle");
     }
-    std::size_t n[4] = {0, 0, 0, 0};
+    std::size_t n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (const std::uint8_t c : h->tables.cls) {
-        ++n[c & 3U];
+        ++n[c & 7U];
     }
-    if (bulk != nullptr) *bulk = n[1];
+    // inlet nodes stream like bulk nodes
+    if (bulk != nullptr) *bulk = n[1] + n[4];
     if (side != nullptr) *side = n[3];
     if (listed != nullptr) *listed = n[2];
     return ok();

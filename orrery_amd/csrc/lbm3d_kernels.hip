@@ -41,6 +41,9 @@ __global__ __launch_bounds__(256) void d3q19_fused(Lattice3<typename T::R> p) {
         R v[19];
         d3::bulk_load<T, kLocal>(p, c, v);
         if (d3::streamed(kind)) {
+            if (kind == kInletNode) {
+                d3::inlet_west3<T>(v, p);
+            }
             d3::bulk_finish<T, kLocal>(p, c, v);  // in the A step a side node streams as a bulk node
         }
     } else {

@@ -251,20 +251,23 @@ __device__ __forceinline__ void inlet_west3(typename T::R (&f)[19], const Lattic
         }
     }
     const R uu = ux * ux + uy * uy + uz * uz;
-    R feq[19];
+    // the equilibrium is evaluated twice, identically, instead of being kept: 19 fewer live
+    // registers where the bulk kernel inlines this for its inlet nodes
+    const auto feq = [&](int i) -> R {
+        const R cu = R(L3::cx(i)) * ux + R(L3::cy(i)) * uy + R(L3::cz(i)) * uz;
+        if constexpr (T::shifted) {
+            return R(L3::w(i)) * (drho + rho * (R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu));
+        } else {
+            return L3::w(i) * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+        }
+    };
     R pi[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const R ex = R(L3::cx(i));
         const R ey = R(L3::cy(i));
         const R ez = R(L3::cz(i));
-        const R cu = ex * ux + ey * uy + ez * uz;
-        if constexpr (T::shifted) {
-            feq[i] = R(L3::w(i)) * (drho + rho * (R(3) * cu + R(4.5) * cu * cu - R(1.5) * uu));
-        } else {
-            feq[i] = L3::w(i) * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
-        }
-        const R neq = f[i] - feq[i];
+        const R neq = f[i] - feq(i);
         pi[0] += ex * ex * neq;
         pi[1] += ey * ey * neq;
         pi[2] += ez * ez * neq;
@@ -278,7 +281,7 @@ __device__ __forceinline__ void inlet_west3(typename T::R (&f)[19], const Lattic
         const R ex = R(L3::cx(i));
         const R ey = R(L3::cy(i));
         const R ez = R(L3::cz(i));
-        f[i] = feq[i] + R(4.5) * R(L3::w(i)) *
+        f[i] = feq(i) + R(4.5) * R(L3::w(i)) *
                             ((ex * ex - cs2) * pi[0] + (ey * ey - cs2) * pi[1] + (ez * ez - cs2) * pi[2] +
                              R(2) * (ex * ey * pi[3] + ex * ez * pi[4] + ey * ez * pi[5]));
     }
@@ -435,8 +438,13 @@ template <class T> __device__ __forceinline__ void side_node_b(const Lattice3<ty
     }
 }
 
-// A bulk or side node: both update in place; they differ only in the B step's addresses.
-__device__ __forceinline__ bool streamed(std::uint8_t kind) { return kind == kBulkNode || kind == kSideNode; }
+// Nodes updated in place by the bulk kernel (side nodes in A steps only; the side kernel updates
+// them in B steps) and read through the lattice by their neighbours.
+__device__ __forceinline__ bool streamed(std::uint8_t kind) {
+    return kind == kBulkNode || kind == kSideNode || kind == kInletNode;
+}
+// Nodes the bulk kernel updates in B steps: their B-step addresses are the plain ones.
+__device__ __forceinline__ bool plain_b(std::uint8_t kind) { return kind == kBulkNode || kind == kInletNode; }
 
 // V consecutive stored populations of one plane, as one memory access.
 template <class T, int V> __device__ __forceinline__ void load_run(const typename T::S* src, typename T::R (&out)[V]) {
@@ -481,15 +489,15 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
         return;
     }
     // The class array is padded with solid (0) entries up to the plane stride. The A step
-    // updates bulk and side nodes (a side node's A step is a bulk node's); the B step bulk nodes
-    // only, side nodes having a kernel of their own then (side_node_b).
+    // updates bulk, side and inlet nodes (a side node's A step is a bulk node's); the B step bulk
+    // and inlet nodes, side nodes having a kernel of their own then (side_node_b).
     std::uint8_t kind[V];
     __builtin_memcpy(kind, p.cls + c0, sizeof(kind));
     bool any = false;
     bool all = true;
 #pragma unroll
     for (int c = 0; c < V; ++c) {
-        const bool mine = kLocal ? streamed(kind[c]) : kind[c] == kBulkNode;
+        const bool mine = kLocal ? streamed(kind[c]) : plain_b(kind[c]);
         any = any || mine;
         all = all && mine;
     }
@@ -519,7 +527,7 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
             v[c][i] = run[c];
         }
     }
-    const auto updated = [&](int c) { return kLocal ? streamed(kind[c]) : kind[c] == kBulkNode; };
+    const auto updated = [&](int c) { return kLocal ? streamed(kind[c]) : plain_b(kind[c]); };
 #pragma unroll
     for (int c = 0; c < V; ++c) {
         if (updated(c)) {
@@ -530,6 +538,11 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
                 const std::uint32_t rest = cell - z * static_cast<std::uint32_t>(plane);
                 const std::uint32_t y = p.div_nx.div(rest);
                 sponge = sponge_sigma3(p, rest - y * p.nx, y, z);
+            }
+            if (kind[c] == kInletNode) {
+                // the pulls from x = -1 (loaded from unowned addresses) are replaced by the
+                // regularised inlet, as the boundary kernel does for a listed inlet node
+                inlet_west3<T>(v[c], p);
             }
             collide3<T>(v[c], p, sponge);
         }

@@ -135,9 +135,12 @@ struct StoreF64 {
 // 0 solid; 1 bulk (all 18 neighbours in the domain and fluid); 2 boundary (listed: faces and
 // neighbours of solids, updated by the boundary kernel through source codes and the stash);
 // 3 side (a fluid node on a y or z face, away from the x faces and from solids, whose every
-// source lies in the lattice after the side's mirror or wrap: streamed by the bulk kernel). The
-// reference's classification (classify_nodes) counts side nodes as boundary nodes.
-enum NodeClass : std::uint8_t { kSolidNode = 0, kBulkNode = 1, kListedNode = 2, kSideNode = 3 };
+// source lies in the lattice after the side's mirror or wrap: streamed by the bulk kernel in A
+// steps and the side kernel in B steps); 4 inlet (a node of the velocity inlet face x = 0, off
+// the y and z faces and away from solids: streamed by the bulk kernel like a bulk node, the
+// regularised inlet rule replacing the pulls from x = -1). The reference's classification
+// (classify_nodes) counts side and inlet nodes as boundary nodes.
+enum NodeClass : std::uint8_t { kSolidNode = 0, kBulkNode = 1, kListedNode = 2, kSideNode = 3, kInletNode = 4 };
 
 // ---- boundary-node source codes -------------------------------------------------------------
 // One 32-bit word per (boundary node, direction), built on the host from resolve_d3q19:

@@ -36,8 +36,8 @@ struct Link {
 
 // Everything the device lattice needs from the mask.
 struct Tables {
-    // 0 solid, 1 bulk, 2 boundary (listed), 3 side (D3Q19: streamed in place by the bulk kernel;
-    // the reference's classification counts them as boundary nodes)
+    // 0 solid, 1 bulk, 2 boundary (listed), 3 side, 4 inlet (D3Q19: streamed in place by the
+    // bulk kernel; the reference's classification counts them as boundary nodes)
     std::vector<std::uint8_t> cls;
     std::vector<std::uint32_t> bcell;    // boundary node -> cell (cell order)
     std::vector<std::uint32_t> scell;    // D3Q19: side node -> cell (cell order)

@@ -345,15 +345,19 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
                     t.cls[here] = 2;
                     // A side node: on a y or z face, not on an x face, and every source in the
                     // lattice after the sides' mirrors and wraps (no wall, solid, inlet or outlet
-                    // source). It streams in place through the bulk kernel.
-                    bool side = face_row && x > 0 && x + 1 < nx;
-                    for (int i = 1; i < 19 && side; ++i) {
+                    // source). An inlet node: on the velocity inlet face, off the y and z faces,
+                    // every source in the lattice or beyond the inlet. Both stream in place
+                    // through the bulk kernel.
+                    const bool side = face_row && x > 0 && x + 1 < nx;
+                    const bool inlet = !face_row && x == 0 && c.x_boundary == LBM_X_INFLOW_OUTFLOW;
+                    bool in_place = side || inlet;
+                    for (int i = 1; i < 19 && in_place; ++i) {
                         const Resolved r = resolve_d3q19(static_cast<int>(x), static_cast<int>(y), static_cast<int>(z), i,
                                                          inx, iny, inz, c.x_boundary, sides, solid_at);
-                        side = r.kind == kNode;
+                        in_place = r.kind == kNode || (inlet && r.kind == kInlet);
                     }
-                    if (side) {
-                        t.cls[here] = 3;
+                    if (in_place) {
+                        t.cls[here] = side ? 3 : 4;
                     } else {
                         ++nb;
                     }
@@ -409,7 +413,8 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
                         // The source's output is also in its stash slot, densely.
                         code = make_code(kStash, my, mz, slot_of[src]);
                     } else {
-                        // A bulk or side node: through the lattice (mirrored only for side nodes).
+                        // A bulk, side or inlet node: through the lattice (mirrored only for side
+                        // nodes).
                         code = make_code(kNode, my, mz, src);
                     }
                     break;
@@ -490,7 +495,7 @@ Tables tables3(const lbm3d_config& c, const std::uint8_t* solid, const double* d
                 continue;
             }
             const std::uint32_t n = cell_of(tx, ty, tz);
-            bool needed = t.cls[n] == 1 || t.cls[n] == 3;
+            bool needed = t.cls[n] == 1 || t.cls[n] == 3 || t.cls[n] == 4;
             if (t.cls[n] == 2 && !mirrored) {
                 const std::uint32_t far = t.bcode[static_cast<std::size_t>(D3Q19::opp(i)) * t.bstride + slot_of[n]];
                 needed = static_cast<int>(far & 7U) == kBounce;
@@ -669,8 +674,8 @@ int lbm3d_classify(const lbm3d_config* config, const uint8_t* solid, uint8_t* cl
         return rc;
     }
     const lbm::host::Tables t = lbm::host::tables3(c, solid, nullptr);
-    // the reference's classes: side nodes are boundary nodes
-    std::transform(t.cls.begin(), t.cls.end(), classes, [](std::uint8_t k) { return k == 3 ? std::uint8_t{2} : k; });
+    // the reference's classes: side and inlet nodes are boundary nodes
+    std::transform(t.cls.begin(), t.cls.end(), classes, [](std::uint8_t k) { return k >= 3 ? std::uint8_t{2} : k; });
     return LBM_OK;
 }
 

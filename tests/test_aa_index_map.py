@@ -115,15 +115,17 @@ def test_b_step_sources_stay_inside_the_guards():
                 assert -lat.guard <= e < lat.cells + lat.guard
 
 
-# ---- side nodes and the boundary nodes around them ------------------------------------------------
+# ---- side and inlet nodes and the boundary nodes around them ---------------------------------------
 # A lattice with slip or periodic y and z sides and an inflow/outflow x: the rows on the y and z
-# faces (away from the x faces) hold side nodes, streamed in place by the bulk kernel with the
-# B-step address of pull i at plane opp(d), element s, (s, d) the source node and direction
-# through the sides; every other face node is a listed boundary node, which reads the outputs
-# of listed sources from their stash, those of bulk and side sources from the lattice (A step:
-# its own cell, plane i; B step: plane opp(d) at s), and stores an output in the lattice when
-# its reader is a bulk or side node (A step: its own cell, plane opp(k); B step: the reader's
-# A-step address, plane d' at r, (r, d') the reader and its pull through the sides).
+# faces (away from the x faces) hold side nodes, streamed in place with the B-step address of
+# pull i at plane opp(d), element s, (s, d) the source node and direction through the sides; the
+# inlet face off the side faces holds inlet nodes, streamed with the plain addresses (their
+# pulls from x = -1 land on addresses no other node touches, and the inlet rule replaces them);
+# every other face node is a listed boundary node, which reads the outputs of listed sources
+# from their stash, those of streamed sources from the lattice (A step: its own cell, plane i;
+# B step: plane opp(d) at s), and stores an output in the lattice when its reader is streamed
+# (A step: its own cell, plane opp(k); B step: the reader's A-step address, plane d' at r,
+# (r, d') the reader and its pull through the sides).
 MY = [0, 1, 2, 4, 3, 5, 6, 9, 10, 7, 8, 11, 12, 13, 14, 18, 17, 16, 15]
 MZ = [0, 1, 2, 3, 4, 6, 5, 7, 8, 9, 10, 13, 14, 11, 12, 17, 18, 15, 16]
 SLIP, PERIODIC = "slip", "periodic"
@@ -141,7 +143,9 @@ class SideLattice:
         for x in range(self.cells):
             xx, yy, zz = self.xyz(x)
             face = yy in (0, ny - 1) or zz in (0, nz - 1)
-            if xx in (0, nx - 1):
+            if xx == 0 and not face:
+                self.cls[x] = 4
+            elif xx in (0, nx - 1):
                 self.cls[x] = 2
             elif face:
                 self.cls[x] = 3 if all(self.resolve(x, i)[0] == "node" for i in range(19)) else 2
@@ -191,8 +195,10 @@ class SideLattice:
         return self.at(ta, tb, tc), d
 
     def b_address(self, x, i):
-        """B-step load address of pull i of a bulk or side node (the store of output opp(i))."""
+        """B-step load address of pull i of a streamed node (the store of output opp(i))."""
         kind, s, d = self.resolve(x, i)
+        if kind == "inlet":  # an inlet node's pull from x = -1: the plain address
+            return (OPP[i], x - CX[i] - self.nx * CY[i] - self.nx * self.ny * CZ[i])
         assert kind == "node"
         if self.cls[x] == 3 and not self.mirror_side_loads:
             return (OPP[i], x - CX[i] - self.nx * CY[i] - self.nx * self.ny * CZ[i])
@@ -203,7 +209,7 @@ class SideLattice:
 
 
 def run_sides(lat, steps):
-    streamed = [x for x in range(lat.cells) if lat.cls[x] in (1, 3)]
+    streamed = [x for x in range(lat.cells) if lat.cls[x] in (1, 3, 4)]
     listed = [x for x in range(lat.cells) if lat.cls[x] == 2]
     for x in range(lat.cells):
         for i in range(19):
@@ -214,6 +220,7 @@ def run_sides(lat, steps):
     for n in range(steps):
         local = n % 2 == 1
         written, pulled = {}, {}
+        read_by, writer = {}, {}  # lattice address -> the node reading / writing it this step
         for x in streamed:
             if local:
                 loads = [(i, x) for i in range(19)]
@@ -224,14 +231,16 @@ def run_sides(lat, steps):
             if sorted(loads) != sorted(stores):
                 failures.append(("not in place", x, n))
             for _, e in loads:
-                if not 0 <= e < lat.cells:
-                    failures.append(("outside the planes", x, n))
+                if not -(lat.nx * lat.ny + lat.nx + 1) <= e < lat.cells + lat.nx * lat.ny + lat.nx + 1:
+                    failures.append(("outside the guards", x, n))
             for i, a in enumerate(loads):
                 pulled[(x, i)] = lat.mem.get(a)
+                read_by.setdefault(a, set()).add(x)
             for k, a in enumerate(stores):
                 if a in written:
                     failures.append(("written twice", a, n))
                 written[a] = (x, k, n + 1)
+                writer[a] = x
         new_stash = {}
         for x in listed:
             for i in range(19):
@@ -241,7 +250,9 @@ def run_sides(lat, steps):
                 if lat.cls[s] == 2:
                     pulled[(x, i)] = lat.stash[(s, d)]
                 else:
-                    pulled[(x, i)] = lat.mem.get((i, x) if local else (OPP[d], s))
+                    a = (i, x) if local else (OPP[d], s)
+                    pulled[(x, i)] = lat.mem.get(a)
+                    read_by.setdefault(a, set()).add(x)
             for k in range(19):
                 new_stash[(x, k)] = (x, k, n + 1)
                 r = lat.reader(x, k)
@@ -251,6 +262,12 @@ def run_sides(lat, steps):
                 if a in written:
                     failures.append(("written twice", a, n))
                 written[a] = (x, k, n + 1)
+                writer[a] = x
+        # the kernels of a step run one after another: an address one node writes must not be
+        # read by another node in the same step
+        for a, w in writer.items():
+            if read_by.get(a, {w}) - {w}:
+                failures.append(("read and written by different nodes", a, n))
         for x in streamed + listed:
             for i in range(19):
                 kind, s, d = lat.resolve(x, i)
@@ -270,7 +287,7 @@ def run_sides(lat, steps):
 @pytest.mark.parametrize("dims", [(6, 5, 4), (5, 3, 6)])
 def test_side_nodes_stream_in_place_through_slip_and_periodic_sides(dims, sides):
     lat = SideLattice(*dims, sides)
-    assert any(c == 3 for c in lat.cls.values())
+    assert any(c == 3 for c in lat.cls.values()) and any(c == 4 for c in lat.cls.values())
     assert run_sides(lat, 6) == []
 
 
