@@ -290,6 +290,7 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
         ((h->tile_slot = static_cast<std::uint32_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, std::max<std::size_t>(t.scell.size(), 1) * 4U, "side list")) ||
         ((h->scell = static_cast<std::uint32_t*>(tmp)), false) ||
+        (rc = alloc(*h, &h->offsets, 38U * rb, "storage offsets")) ||
         (rc = alloc(*h, &tmp, 19U * static_cast<std::size_t>(t.bstride) * 4U, "source codes")) ||
         ((h->bcode = static_cast<std::uint32_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, static_cast<std::size_t>(t.bstride) * 4U, "lattice write masks")) ||
@@ -320,6 +321,14 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
         ldir[k] = static_cast<std::uint8_t>(t.links[k].direction);
         lq[k] = t.links[k].q;
     }
+    // Storage offsets of the 16-bit formats (zeros for the others), as bytes of the arithmetic type;
+    // kept until the stream is drained below.
+    std::vector<unsigned char> offsets(38U * rb);
+    (void)with_store(*h, [&]<class T>() -> int {
+        const std::vector<typename T::R> v = lbm::internal::storage_offsets<T>(h->cfg);
+        std::memcpy(offsets.data(), v.data(), offsets.size());
+        return LBM_OK;
+    });
     // The class array is padded to the plane stride with solid entries, so the bulk kernel
     // reads the classes of a whole run of cells at once.
     if (stride > cells) {
@@ -332,6 +341,7 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     if ((rc = upload(*h, h->cls, t.cls, "node class upload")) || (rc = upload(*h, h->bcell, t.bcell, "boundary upload")) ||
         (rc = upload(*h, h->tile_slot, t.tile_slot, "boundary slot upload")) ||
         (rc = upload(*h, h->scell, t.scell, "side list upload")) ||
+        (rc = upload(*h, h->offsets, offsets, "storage offsets upload")) ||
         (rc = upload(*h, h->bcode, t.bcode, "code upload")) || (rc = upload(*h, h->bwrite, t.bwrite, "mask upload")) ||
         (rc = upload(*h, h->lcell, lcell, "link upload")) ||
         (rc = upload(*h, h->ldir, ldir, "link upload")) || (rc = upload(*h, h->lq, lq, "link upload")) ||

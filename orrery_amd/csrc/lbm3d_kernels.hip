@@ -133,6 +133,9 @@ __global__ __launch_bounds__(256) void d3q19_init(Lattice3<typename T::R> p, int
         if constexpr (T::shifted) {
             value -= L3::w(i);
         }
+        if constexpr (T::offset) {
+            value -= static_cast<double>(p.shift[i]);
+        }
         f[L3::opp(i) * p.stride + cell] = T::from_double(value);
     }
 }
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(256) void d3q19_init_boundary(Lattice3<typename T::
     const std::uint32_t cell = p.bcell[b];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        const typename T::R v = T::load(f[L3::opp(i) * p.stride + cell]);
+        const typename T::R v = d3::decode<T>(p, f[L3::opp(i) * p.stride + cell], i);
         p.stash[i * p.bstride + b] = v;
         if (L3::cx(i) < 0) {
             p.hist[L3::outlet_slot(i) * p.bstride + b] = v;
@@ -165,13 +168,13 @@ template <class T>
 __device__ __forceinline__ typename T::R bulk_value(const Lattice3<typename T::R>& p, const typename T::S* f,
                                                     int i, std::uint32_t cell, int phase, std::uint8_t kind) {
     if (phase == 0) {
-        return T::load(f[L3::opp(i) * p.stride + cell]);
+        return d3::decode<T>(p, f[L3::opp(i) * p.stride + cell], i);
     }
     if (kind == kSideNode) {
-        return T::load(f[d3::side_element(p, cell, d3::side_faces(p, cell), L3::opp(i), 0U)]);
+        return d3::decode<T>(p, f[d3::side_element(p, cell, d3::side_faces(p, cell), L3::opp(i), 0U)], i);
     }
     const int off = off3(i, static_cast<int>(p.nx), static_cast<int>(p.nx * p.ny));
-    return T::load(f[i * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off)]);
+    return d3::decode<T>(p, f[i * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off)], i);
 }
 
 // Populations with the weights added back, [19][cells] doubles. Pass 1 over cells covers
@@ -354,9 +357,10 @@ __global__ __launch_bounds__(256) void d3q19_force(Lattice3<typename T::R> p, in
             double farther;
             if (phase == 0) {
                 const int off = off3(j, static_cast<int>(p.nx), static_cast<int>(p.nx * p.ny));
-                farther = static_cast<double>(T::load(f[j * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off)]));
+                farther = static_cast<double>(
+                    d3::decode<T>(p, f[j * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off)], o));
             } else {
-                farther = static_cast<double>(T::load(f[o * p.stride + cell]));
+                farther = static_cast<double>(d3::decode<T>(p, f[o * p.stride + cell], o));
             }
             if constexpr (T::shifted) {
                 farther += L3::w(o);

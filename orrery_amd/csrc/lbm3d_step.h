@@ -26,6 +26,35 @@ __device__ __forceinline__ int off3(int i, int nx, int plane) {
     return L3::cx(i) + nx * L3::cy(i) + plane * L3::cz(i);
 }
 
+// Entry i of the storage offset table. The table is read-only while kernels run; reading it
+// through the constant address space lets a uniform read become a scalar load, where a plain
+// global pointer gives a vector load per thread (the kernel's own stores might alias it).
+template <class R>
+__device__ __forceinline__ R storage_offset(const Lattice3<R>& p, int i) {
+    using ConstantR = const __attribute__((address_space(4))) R;
+    return ((ConstantR*)p.shift)[i];
+}
+
+// A stored population of direction `dir` as the arithmetic value (f - w for the shifted formats),
+// and back: formats with `offset` store it less shift[dir] (Lattice3::shift).
+template <class T>
+__device__ __forceinline__ typename T::R decode(const Lattice3<typename T::R>& p, typename T::S s, int dir) {
+    if constexpr (T::offset) {
+        return T::load(s, storage_offset(p, dir));
+    } else {
+        return T::load(s);
+    }
+}
+template <class T>
+__device__ __forceinline__ typename T::S encode(const Lattice3<typename T::R>& p, typename T::R v, int dir,
+                                                std::uint32_t dither) {
+    if constexpr (T::offset) {
+        return T::pack(v, storage_offset(p, 19 + dir), dither);
+    } else {
+        return T::pack(v, dither);
+    }
+}
+
 // Relaxation rate of the absorbing layers at (x, y, z) (sponge_sigma,
 // src/aero/lbm3d/lbm3d_internal.h:256-281).
 template <typename R>
@@ -309,7 +338,7 @@ __device__ __forceinline__ void bulk_load(const Lattice3<typename T::R>& p, std:
     if constexpr (kLocal) {
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-            v[i] = T::load(f[i * stride + cell]);
+            v[i] = decode<T>(p, f[i * stride + cell], i);
         }
     } else {
         const std::uint32_t g = p.nx * p.ny + p.nx + 1U;
@@ -317,7 +346,7 @@ __device__ __forceinline__ void bulk_load(const Lattice3<typename T::R>& p, std:
         const std::uint32_t shifted = cell + g;
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-            v[i] = T::load(base[L3::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(shifted) - off3(i, nx, plane))]);
+            v[i] = decode<T>(p, base[L3::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(shifted) - off3(i, nx, plane))], i);
         }
     }
 }
@@ -343,10 +372,10 @@ __device__ __forceinline__ void bulk_finish(const Lattice3<typename T::R>& p, st
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         if constexpr (kLocal) {
-            f[L3::opp(i) * stride + cell] = T::pack(v[i], direction_dither(bits, i));
+            f[L3::opp(i) * stride + cell] = encode<T>(p, v[i], i, direction_dither(bits, i));
         } else {
             f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] =
-                T::pack(v[i], direction_dither(bits, i));
+                encode<T>(p, v[i], i, direction_dither(bits, i));
         }
     }
 }
@@ -421,7 +450,7 @@ template <class T> __device__ __forceinline__ void side_node_b(const Lattice3<ty
     R v[19];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        v[i] = T::load(f[side_element(p, cell, faces, i, 0U)]);
+        v[i] = decode<T>(p, f[side_element(p, cell, faces, i, 0U)], i);
     }
     R sponge = 0;
     if (p.sponge_strength > R(0)) {
@@ -434,7 +463,7 @@ template <class T> __device__ __forceinline__ void side_node_b(const Lattice3<ty
     const std::uint32_t bits = T::dithered ? cell_dither(cell, slot_dither(*p.clock)) : 0U;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        f[side_element(p, cell, faces, L3::opp(i), 0U)] = T::pack(v[i], direction_dither(bits, i));
+        f[side_element(p, cell, faces, L3::opp(i), 0U)] = encode<T>(p, v[i], i, direction_dither(bits, i));
     }
 }
 
@@ -446,24 +475,26 @@ __device__ __forceinline__ bool streamed(std::uint8_t kind) {
 // Nodes the bulk kernel updates in B steps: their B-step addresses are the plain ones.
 __device__ __forceinline__ bool plain_b(std::uint8_t kind) { return kind == kBulkNode || kind == kInletNode; }
 
-// V consecutive stored populations of one plane, as one memory access.
-template <class T, int V> __device__ __forceinline__ void load_run(const typename T::S* src, typename T::R (&out)[V]) {
+// V consecutive stored populations of direction i in one plane, as one memory access.
+template <class T, int V>
+__device__ __forceinline__ void load_run(const Lattice3<typename T::R>& p, const typename T::S* src, int i,
+                                         typename T::R (&out)[V]) {
     typename T::S tmp[V];
     __builtin_memcpy(tmp, src, sizeof(tmp));
 #pragma unroll
     for (int k = 0; k < V; ++k) {
-        out[k] = T::load(tmp[k]);
+        out[k] = decode<T>(p, tmp[k], i);
     }
 }
 
 // Direction i of V consecutive cells, whose dither bits are `bits`.
 template <class T, int V>
-__device__ __forceinline__ void store_run(typename T::S* dst, const typename T::R (&in)[V], const std::uint32_t (&bits)[V],
-                                          int i) {
+__device__ __forceinline__ void store_run(const Lattice3<typename T::R>& p, typename T::S* dst, const typename T::R (&in)[V],
+                                          const std::uint32_t (&bits)[V], int i) {
     typename T::S tmp[V];
 #pragma unroll
     for (int k = 0; k < V; ++k) {
-        tmp[k] = T::pack(in[k], direction_dither(bits[k], i));
+        tmp[k] = encode<T>(p, in[k], i, direction_dither(bits[k], i));
     }
     __builtin_memcpy(dst, tmp, sizeof(tmp));
 }
@@ -516,11 +547,11 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
     for (int i = 0; i < 19; ++i) {
         R run[V];
         if constexpr (kLocal) {
-            load_run<T, V>(element(f + static_cast<std::size_t>(i) * stride, c0), run);
+            load_run<T, V>(p, element(f + static_cast<std::size_t>(i) * stride, c0), i, run);
         } else {
-            load_run<T, V>(element(f + static_cast<std::size_t>(L3::opp(i)) * stride - g,
+            load_run<T, V>(p, element(f + static_cast<std::size_t>(L3::opp(i)) * stride - g,
                                    static_cast<std::uint32_t>(static_cast<int>(c0 + g) - off3(i, nx, plane))),
-                           run);
+                           i, run);
         }
 #pragma unroll
         for (int c = 0; c < V; ++c) {
@@ -564,9 +595,9 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
                 run[c] = v[c][i];
             }
             if constexpr (kLocal) {
-                store_run<T, V>(element(f + static_cast<std::size_t>(L3::opp(i)) * stride, c0), run, bits, i);
+                store_run<T, V>(p, element(f + static_cast<std::size_t>(L3::opp(i)) * stride, c0), run, bits, i);
             } else {
-                store_run<T, V>(element(f + static_cast<std::size_t>(i) * stride,
+                store_run<T, V>(p, element(f + static_cast<std::size_t>(i) * stride,
                                         static_cast<std::uint32_t>(static_cast<int>(c0) + off3(i, nx, plane))),
                                 run, bits, i);
             }
@@ -579,10 +610,10 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
                     if constexpr (kLocal) {
-                        f[L3::opp(i) * stride + cell] = T::pack(v[c][i], direction_dither(bits[c], i));
+                        f[L3::opp(i) * stride + cell] = encode<T>(p, v[c][i], i, direction_dither(bits[c], i));
                     } else {
                         f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] =
-                            T::pack(v[c][i], direction_dither(bits[c], i));
+                            encode<T>(p, v[c][i], i, direction_dither(bits[c], i));
                     }
                 }
             }
@@ -596,6 +627,24 @@ __device__ __forceinline__ std::uint32_t mirrored(int i, std::uint32_t code) {
     return static_cast<std::uint32_t>(pick4(i, L3::mirror_y(i), L3::mirror_z(i), L3::mirror_z(L3::mirror_y(i)),
                                             (code & kCodeMirrorY) != 0U, (code & kCodeMirrorZ) != 0U));
 }
+// A stored population read as pull i through the mirrors of `code`: it was stored as the
+// source's output of direction mirrored(i, code), so that direction's offset applies (the four
+// offsets are selected, not indexed, so that no lane indexes the descriptor).
+template <class T>
+__device__ __forceinline__ typename T::R decode_mirrored(const Lattice3<typename T::R>& p, typename T::S s, int i,
+                                                         std::uint32_t code) {
+    if constexpr (T::offset) {
+        const bool my = (code & kCodeMirrorY) != 0U;
+        const bool mz = (code & kCodeMirrorZ) != 0U;
+        const typename T::R shift =
+            my ? (mz ? storage_offset(p, L3::mirror_z(L3::mirror_y(i))) : storage_offset(p, L3::mirror_y(i)))
+               : (mz ? storage_offset(p, L3::mirror_z(i)) : storage_offset(p, i));
+        return T::load(s, shift);
+    } else {
+        return T::load(s);
+    }
+}
+
 __device__ __forceinline__ std::uint32_t mirrored_opp(int i, std::uint32_t code) {
     return static_cast<std::uint32_t>(pick4(L3::opp(i), L3::opp(L3::mirror_y(i)), L3::opp(L3::mirror_z(i)),
                                             L3::opp(L3::mirror_z(L3::mirror_y(i))), (code & kCodeMirrorY) != 0U,
@@ -693,7 +742,7 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
         const int kind = static_cast<int>(code[i] & 7U);
         const std::uint32_t payload = code[i] >> kCodeShift;
         if (kind == kNode) {
-            v[i] = T::load(fv[i]);
+            v[i] = decode_mirrored<T>(p, fv[i], i, code[i]);
         } else if (kind == kStash) {
             v[i] = sv[i];
         } else if (kind == kBounce) {
@@ -703,7 +752,8 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
             if (qi == R(0.5)) {
                 v[i] = leaving;
             } else if (qi < R(0.5)) {
-                v[i] = R(2) * qi * leaving + (R(1) - R(2) * qi) * T::load(fv[i]);
+                // the farther node's output opp(i)
+                v[i] = R(2) * qi * leaving + (R(1) - R(2) * qi) * decode<T>(p, fv[i], L3::opp(i));
             } else {
                 v[i] = (leaving + (R(2) * qi - R(1)) * own[i]) / (R(2) * qi);
             }
@@ -730,8 +780,8 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
     const std::uint32_t wmask = p.bwrite[b];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        const S packed = T::pack(v[i], direction_dither(bits, i));
-        sw[i * bs + b] = T::load(packed);
+        const S packed = encode<T>(p, v[i], i, direction_dither(bits, i));
+        sw[i * bs + b] = decode<T>(p, packed, i);
         if (((wmask >> i) & 1U) == 0U) {
             continue;
         }

@@ -26,8 +26,10 @@
 namespace lbm {
 
 // ---- storage formats ----------------------------------------------------------------------
-// Each format: S the stored type, R the arithmetic type, `shifted` whether f - w is stored.
-// pack(v, dither) rounds a post-collision value for storage with 32 random bits.
+// Each format: S the stored type, R the arithmetic type, `shifted` whether the arithmetic works
+// on f - w. pack(v, dither) rounds a post-collision value for storage with 32 random bits.
+// `offset`: in the D3Q19 lattices the format stores a population's offset from the free-stream
+// equilibrium rather than f - w itself (Lattice3::shift; the D2Q9 kernels store f - w).
 //
 // The 16-bit formats round stochastically: the bits below the stored significand are raised by
 // the dither and then cut, an unbiased rounding. Round-to-nearest makes the rounding error a
@@ -65,6 +67,7 @@ struct StoreF32 {
     using R = float;
     static constexpr bool shifted = true;
     static constexpr bool dithered = false;
+    static constexpr bool offset = false;
     static constexpr int id = 0;
     static __device__ __forceinline__ R load(S v) { return v; }
     static __device__ __forceinline__ S pack(R v, std::uint32_t) { return v; }
@@ -79,6 +82,7 @@ struct StoreF16 {
     using R = float;
     static constexpr bool shifted = true;
     static constexpr bool dithered = true;
+    static constexpr bool offset = true;
     static constexpr int id = 1;
     static constexpr float kScale = 32768.0f;
     static __device__ __forceinline__ R load(S v) { return static_cast<float>(v) * (1.0f / kScale); }
@@ -94,6 +98,18 @@ struct StoreF16 {
     static __device__ __forceinline__ S from_double(double v) {
         return static_cast<_Float16>(static_cast<float>(v * 32768.0));
     }
+    // With an offset: v = s / 2^15 + offset, one fused multiply-add (the power-of-two scale is
+    // exact, so this is the rounding of the plain sum); stored: (v - offset) 2^15 as
+    // v 2^15 - offset 2^15, `scaled` = offset 2^15, the same rounding.
+    static __device__ __forceinline__ R load(S v, R offset) {
+        return __builtin_fmaf(static_cast<float>(v), 1.0f / kScale, offset);
+    }
+    static __device__ __forceinline__ S pack(R v, R scaled, std::uint32_t dither) {
+        const std::uint32_t u = __float_as_uint(__builtin_fmaf(v, kScale, -scaled));
+        const std::uint32_t d = (u + (dither & 0x1FFFU)) & ~0x1FFFU;
+        return static_cast<_Float16>(__uint_as_float(d));
+    }
+    static constexpr float kOffsetScale = kScale;
 };
 
 // bfloat16 holding f - w, rounded to nearest even.
@@ -102,6 +118,7 @@ struct StoreBF16 {
     using R = float;
     static constexpr bool shifted = true;
     static constexpr bool dithered = true;
+    static constexpr bool offset = true;
     static constexpr int id = 2;
     static __device__ __forceinline__ R load(S v) {
         return __uint_as_float(static_cast<std::uint32_t>(v) << 16U);
@@ -116,6 +133,10 @@ struct StoreBF16 {
     }
     // Initial states round to nearest even.
     static __device__ __forceinline__ S from_double(double v) { return nearest(static_cast<float>(v)); }
+    // With an offset (no scale: `scaled` is the offset itself).
+    static __device__ __forceinline__ R load(S v, R offset) { return load(v) + offset; }
+    static __device__ __forceinline__ S pack(R v, R scaled, std::uint32_t dither) { return pack(v - scaled, dither); }
+    static constexpr float kOffsetScale = 1.0f;
 };
 
 // binary64, unshifted: the on-device oracle, with the reference CPU solver's arithmetic.
@@ -124,6 +145,7 @@ struct StoreF64 {
     using R = double;
     static constexpr bool shifted = false;
     static constexpr bool dithered = false;
+    static constexpr bool offset = false;
     static constexpr int id = 3;
     static __device__ __forceinline__ R load(S v) { return v; }
     static __device__ __forceinline__ S pack(R v, std::uint32_t) { return v; }
@@ -190,6 +212,13 @@ template <typename R> struct Lattice3 {
     R sponge_strength;
     std::uint32_t sponge_in, sponge_out, sponge_sides;
     int sponge_y, sponge_z;
+    // Formats with `offset` store f_i - f_eq_i(1, u_ref): shift[i] = f_eq_i(1, u_ref) - w_i,
+    // with u_ref the configured inlet velocity (zero on a periodic lattice), is added to a loaded
+    // value and taken from a stored one (shift[19 + i]: the same in the stored units, scaled as the
+    // format scales). The uniform stream then stores values near zero, where a 16-bit format's
+    // relative precision is a small absolute error. A device table [38], not descriptor fields:
+    // a lane-dependent index into the descriptor would copy it to scratch memory.
+    const R* shift;
 };
 
 template <typename R> struct Lattice2 {

@@ -60,6 +60,7 @@ struct lbm3d {
     std::uint32_t* lslot = nullptr;
     std::uint32_t* tile_slot = nullptr;
     std::uint32_t* scell = nullptr;
+    void* offsets = nullptr;  // storage offsets, [38] arithmetic-type values (Lattice3::shift)
     double* partial = nullptr;
     double* result = nullptr;
     // Steps since initialisation, on the device (after the reduction results): the 16-bit
@@ -125,6 +126,7 @@ inline std::size_t lattice_bytes(std::size_t cells, std::uint32_t stride, std::s
     total += t.bcell.size() * 4U;                                       // boundary list
     total += t.tile_slot.size() * 4U;                                   // boundary slot per 64 cells
     total += t.scell.size() * 4U;                                       // side list
+    total += 38U * rb;                                                  // storage offsets
     total += 19U * static_cast<std::size_t>(t.bstride) * 4U;            // source codes
     total += static_cast<std::size_t>(t.bstride) * 4U;                  // lattice write masks
     if (interpolated) {
@@ -146,6 +148,25 @@ inline std::size_t required_bytes3(const lbm3d_config& c, const std::uint8_t* so
     const lbm::host::Tables t = lbm::host::tables3(c, solid, distance);
     return lattice_bytes(cells, plane_stride(cells), guard_cells(c.nx, c.ny), t, storage,
                          c.solid_wall == LBM_SOLID_INTERPOLATED);
+}
+
+// The storage offsets of a format with `offset` (Lattice3::shift): f_eq_i(1, u_ref) - w_i with
+// u_ref the configured inlet velocity (the stream the lattice is started in and relaxes to; zero
+// on a periodic lattice), then the same in the stored units. Zero for the other formats.
+template <class T> std::vector<typename T::R> storage_offsets(const lbm3d_config& c) {
+    using R = typename T::R;
+    std::vector<R> t(38, R(0));
+    if constexpr (T::offset) {
+        const bool inflow = c.x_boundary == LBM_X_INFLOW_OUTFLOW;
+        const double u[3] = {inflow ? c.inlet_ux : 0.0, inflow ? c.inlet_uy : 0.0, inflow ? c.inlet_uz : 0.0};
+        const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        for (int i = 0; i < 19; ++i) {
+            const double cu = lbm::D3Q19::cx(i) * u[0] + lbm::D3Q19::cy(i) * u[1] + lbm::D3Q19::cz(i) * u[2];
+            t[i] = static_cast<R>(lbm::D3Q19::w(i) * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu));
+            t[19 + i] = t[i] * T::kOffsetScale;  // exact: a power of two
+        }
+    }
+    return t;
 }
 
 template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
@@ -207,6 +228,7 @@ template <class T> lbm::Lattice3<typename T::R> descriptor3(const lbm3d& h) {
     p.sponge_sides = static_cast<std::uint32_t>(c.sponge_sides);
     p.sponge_y = c.y_low.kind != LBM_WALL_PERIODIC ? 1 : 0;
     p.sponge_z = c.z_low.kind != LBM_WALL_PERIODIC ? 1 : 0;
+    p.shift = static_cast<const R*>(h.offsets);
     return p;
 }
 

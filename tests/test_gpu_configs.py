@@ -17,10 +17,12 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-# Error bars against the FP64 lattice at step 2000 (drag force relative error, velocity L2
-# relative error over fluid nodes). f16s holds the drag within 0.5% (measured 1.6e-4); bf16s
-# keeps 8 significand bits and is held to its measured level (1.8e-2), stated, not the 1% bar.
-C3_BOUNDS = {"f32": (1e-3, 1e-4), "f16s": (5e-3, 2e-3), "bf16s": (5e-2, 1e-2)}
+# Error bars against the FP64 lattice at step 2000: (drag force relative error, velocity L2
+# relative error over fluid nodes). Measured 3.9e-4 / 3.5e-5 (f32), 2.1e-3 / 7.1e-5 (f16s) and
+# 3.4e-3 / 4.9e-4 (bf16s). A drag sample in a 16-bit format carries the rounding noise of one
+# step's link populations: sampled every 100 steps to step 2000 its error reaches 3.1e-3 (f16s)
+# and 2.1e-2 (bf16s), while the mean over steps 1100-2000 is within 1.4e-4 and 3.4e-3.
+C3_BOUNDS = {"f32": (1e-3, 1e-4), "f16s": (5e-3, 5e-4), "bf16s": (3e-2, 2e-3)}
 
 
 @pytest.fixture(scope="module")

@@ -365,8 +365,12 @@ def test_refusals_happen_before_allocation():
         set_memory_limit(0)
 
 
-@pytest.mark.parametrize("storage,limit", [("f16s", 5e-3), ("bf16s", 5e-2)])
-def test_sixteen_bit_storage_error_against_fp64(storage, limit):
+# The velocity field is held tightly (measured f16s 4.2e-5, bf16s 3.3e-4). A force sample reads
+# the link populations of one step, each carrying its stochastic rounding: the force error of
+# this case moves with random sign between samples, up to 8e-4 (f16s) and 1e-2 (bf16s) over
+# steps 100-6400, so its bar is that noise level.
+@pytest.mark.parametrize("storage,velocity_limit,force_limit", [("f16s", 2e-4, 5e-3), ("bf16s", 2e-3, 5e-2)])
+def test_sixteen_bit_storage_error_against_fp64(storage, velocity_limit, force_limit):
     variants, mask = sphere_variants()
     c = variants["bgk"]
     ref = Lbm3DSolver(c, mask, "f64")
@@ -376,5 +380,5 @@ def test_sixteen_bit_storage_error_against_fp64(storage, limit):
     v = velocity_l2(low.macroscopic(), ref.macroscopic())
     f = abs(low.force().force[0] - ref.force().force[0]) / abs(ref.force().force[0])
     print(f"measured: {storage} vs f64 sphere Re 20 after 800 steps: velocity L2 {v:.3e}, force_x {f:.3e}")
-    assert v < limit
-    assert f < limit
+    assert v < velocity_limit
+    assert f < force_limit
