@@ -206,11 +206,13 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
         return fail(LBM_EINVAL, p);
     }
     const std::size_t cells = c.nx * c.ny * c.nz;
-    const std::uint32_t stride = lbm::internal::plane_stride(cells);
-    // Every index the kernels form (population i * stride + cell) must fit in 32 bits
-    // (lbm3d_gpu.cpp:492-497).
-    if (static_cast<std::uint64_t>(stride) * 19U + 2U * lbm::internal::guard_cells(c.nx, c.ny) >
-        std::numeric_limits<std::uint32_t>::max()) {
+    const lbm::internal::PopulationLayout layout = lbm::internal::population_layout(c.nx, c.ny, c.nz, storage);
+    // Every population index the kernels form must fit in 32 bits (lbm3d_gpu.cpp:492-497), and
+    // the byte offsets of the bulk kernel's waves (a few blocks either side of a plane's
+    // neighbours, at most 2 (nx ny + nx + 1) + 4 2^shift cells of 19 populations) too.
+    const std::uint64_t wave_span = (2U * static_cast<std::uint64_t>(layout.guard) + (std::uint64_t{4} << layout.shift)) * 19U *
+                                    static_cast<std::uint64_t>(store_bytes(storage));
+    if (layout.elements > std::numeric_limits<std::uint32_t>::max() || wave_span > std::numeric_limits<std::uint32_t>::max()) {
         return fail(LBM_EINVAL, "lbm3d gpu: lattice too large for 32-bit indices");
     }
     if (storage != LBM_STORE_F64) {
@@ -227,13 +229,12 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     h->cpt = lbm::internal::default_cells_per_thread(storage);
     h->cpt_b = h->cpt;
     h->cells = cells;
-    h->stride = stride;
+    h->layout = layout;
     h->solid.assign(solid, solid + cells);
     h->tables = lbm::host::tables3(c, solid, distance);
     const bool interpolated = c.solid_wall == LBM_SOLID_INTERPOLATED;
     // check_memory (lbm3d_gpu.cpp:131-149): refuse before allocating.
-    const std::size_t guard = lbm::internal::guard_cells(c.nx, c.ny);
-    const std::size_t need = lattice_bytes(cells, stride, guard, h->tables, storage, interpolated);
+    const std::size_t need = lattice_bytes(cells, layout, h->tables, storage, interpolated);
     std::size_t available = 0;
     if (int rc = lbm::host::device_memory(&available, nullptr); rc != LBM_OK) {
         delete h;
@@ -281,9 +282,9 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
                                                   6U * lbm::reduction_blocks(nl));
     int rc = LBM_OK;
     void* tmp = nullptr;
-    if ((rc = alloc(*h, &h->f, (19U * static_cast<std::size_t>(stride) + 2U * guard) * sb, "population buffer")) ||
-        ((h->f = static_cast<char*>(h->f) + guard * sb), false) ||
-        (rc = alloc(*h, &tmp, stride, "node class buffer")) || ((h->cls = static_cast<std::uint8_t*>(tmp)), false) ||
+    const std::size_t cls_len = lbm::internal::class_length(cells);
+    if ((rc = alloc(*h, &h->f, layout.elements * sb, "population buffer")) ||
+        (rc = alloc(*h, &tmp, cls_len, "node class buffer")) || ((h->cls = static_cast<std::uint8_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, std::max<std::size_t>(nb, 1) * 4U, "boundary list")) ||
         ((h->bcell = static_cast<std::uint32_t*>(tmp)), false) ||
         (rc = alloc(*h, &tmp, t.tile_slot.size() * 4U, "boundary slot table")) ||
@@ -329,10 +330,10 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
         std::memcpy(offsets.data(), v.data(), offsets.size());
         return LBM_OK;
     });
-    // The class array is padded to the plane stride with solid entries, so the bulk kernel
-    // reads the classes of a whole run of cells at once.
-    if (stride > cells) {
-        const hipError_t pe = hipMemsetAsync(h->cls + cells, 0, stride - cells, h->stream);
+    // The class array is padded with solid entries, so the bulk kernel reads the classes of a
+    // whole run of cells at once.
+    if (cls_len > cells) {
+        const hipError_t pe = hipMemsetAsync(h->cls + cells, 0, cls_len - cells, h->stream);
         if (pe != hipSuccess) {
             release(h);
             return hip_fail("node class padding", pe);
@@ -1109,13 +1110,12 @@ int lbm3d_autotune(lbm3d* h, uint32_t steps, int32_t serial_only, lbm_launch_cho
     const std::uint64_t timed = steps + (steps & 1U); // whole A-B pairs
     // The state a step reads and writes: populations (with their guard cells), the boundary
     // stash and the outlet history. A device copy of it brings the lattice back after timing.
-    const std::size_t guard = lbm::internal::guard_cells(h->cfg.nx, h->cfg.ny);
     const std::size_t sb = store_bytes(h->storage);
     const std::size_t rb = real_bytes(h->storage);
-    const std::size_t f_bytes = (19U * static_cast<std::size_t>(h->stride) + 2U * guard) * sb;
+    const std::size_t f_bytes = h->layout.elements * sb;
     const std::size_t stash_bytes = 2U * 19U * static_cast<std::size_t>(h->tables.bstride) * rb;
     const std::size_t hist_bytes = 5U * static_cast<std::size_t>(h->tables.bstride) * rb;
-    char* f_base = static_cast<char*>(h->f) - guard * sb;
+    char* f_base = static_cast<char*>(h->f);
     void* snap = nullptr;
     if (hipMalloc(&snap, f_bytes + stash_bytes + hist_bytes + sizeof(std::uint32_t)) != hipSuccess) {
         (void)hipGetLastError();

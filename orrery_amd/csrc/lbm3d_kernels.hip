@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void d3q19_init(Lattice3<typename T::R> p, int
         if constexpr (T::offset) {
             value -= static_cast<double>(p.shift[i]);
         }
-        f[L3::opp(i) * p.stride + cell] = T::from_double(value);
+        f[d3::pop(p, L3::opp(i), cell)] = T::from_double(value);
     }
 }
 
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void d3q19_init_boundary(Lattice3<typename T::
     const std::uint32_t cell = p.bcell[b];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        const typename T::R v = d3::decode<T>(p, f[L3::opp(i) * p.stride + cell], i);
+        const typename T::R v = d3::decode<T>(p, f[d3::pop(p, L3::opp(i), cell)], i);
         p.stash[i * p.bstride + b] = v;
         if (L3::cx(i) < 0) {
             p.hist[L3::outlet_slot(i) * p.bstride + b] = v;
@@ -168,13 +168,13 @@ template <class T>
 __device__ __forceinline__ typename T::R bulk_value(const Lattice3<typename T::R>& p, const typename T::S* f,
                                                     int i, std::uint32_t cell, int phase, std::uint8_t kind) {
     if (phase == 0) {
-        return d3::decode<T>(p, f[L3::opp(i) * p.stride + cell], i);
+        return d3::decode<T>(p, f[d3::pop(p, L3::opp(i), cell)], i);
     }
     if (kind == kSideNode) {
-        return d3::decode<T>(p, f[d3::side_element(p, cell, d3::side_faces(p, cell), L3::opp(i), 0U)], i);
+        return d3::decode<T>(p, f[d3::side_element(p, cell, d3::side_faces(p, cell), L3::opp(i))], i);
     }
     const int off = off3(i, static_cast<int>(p.nx), static_cast<int>(p.nx * p.ny));
-    return d3::decode<T>(p, f[i * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off)], i);
+    return d3::decode<T>(p, f[d3::pop(p, i, cell, off)], i);
 }
 
 // Populations with the weights added back, [19][cells] doubles. Pass 1 over cells covers
@@ -358,9 +358,9 @@ __global__ __launch_bounds__(256) void d3q19_force(Lattice3<typename T::R> p, in
             if (phase == 0) {
                 const int off = off3(j, static_cast<int>(p.nx), static_cast<int>(p.nx * p.ny));
                 farther = static_cast<double>(
-                    d3::decode<T>(p, f[j * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off)], o));
+                    d3::decode<T>(p, f[d3::pop(p, j, cell, off)], o));
             } else {
-                farther = static_cast<double>(d3::decode<T>(p, f[o * p.stride + cell], o));
+                farther = static_cast<double>(d3::decode<T>(p, f[d3::pop(p, o, cell)], o));
             }
             if constexpr (T::shifted) {
                 farther += L3::w(o);

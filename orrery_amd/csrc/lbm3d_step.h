@@ -26,6 +26,49 @@ __device__ __forceinline__ int off3(int i, int nx, int plane) {
     return L3::cx(i) + nx * L3::cy(i) + plane * L3::cz(i);
 }
 
+// ---- population addresses (the blocked layout of Lattice3::f) ----------------------------------
+// Element of A[dir][c] for the guarded cell index g = c + guard.
+template <class R>
+__device__ __forceinline__ std::uint32_t guarded_element(const Lattice3<R>& p, std::uint32_t g, std::uint32_t dir) {
+    const std::uint32_t k = p.block_shift;
+    return (__umul24(g >> k, 19U) + dir) << k | (g & ((1U << k) - 1U));
+}
+// Element of A[dir][cell + off] (a neighbour of a cell, or any cell within the guards).
+template <class R>
+__device__ __forceinline__ std::uint32_t pop(const Lattice3<R>& p, int dir, std::uint32_t cell, int off = 0) {
+    return guarded_element(p, static_cast<std::uint32_t>(static_cast<int>(cell + p.guard) + off),
+                           static_cast<std::uint32_t>(dir));
+}
+
+// The blocked layout seen from one wave: a uniform base at block `first` (the wave's first
+// active lane's block, moved back by the guard blocks) and per-lane element offsets from it.
+// A wave's cells and their neighbours span a few blocks around its own, so the offsets stay far
+// below 2^32 bytes and every access is a scalar 64-bit base plus a 32-bit lane offset. Lanes must
+// hold increasing cells (the first active lane then has the lowest block).
+template <class R> struct WaveBlocks {
+    std::uint32_t first;  // uniform block number of the base
+    std::uint32_t mask;   // 2^k - 1
+    std::uint32_t shift;  // k
+    // Element offset of A[dir][c] from the base, g = c + guard.
+    __device__ __forceinline__ std::uint32_t offset(std::uint32_t g, std::uint32_t dir) const {
+        return (__umul24((g >> shift) - first, 19U) + dir) << shift | (g & mask);
+    }
+    // The element offset of A[0][c]; direction dir lies dir 2^k further (a uniform step).
+    __device__ __forceinline__ std::uint32_t offset0(std::uint32_t g) const {
+        return __umul24((g >> shift) - first, 19U) << shift | (g & mask);
+    }
+};
+// The wave's blocks for a lane whose guarded cell is g0 (every lane of the wave active here);
+// `back` blocks before the first lane's block (the guard blocks, for shifted addresses).
+template <class R>
+__device__ __forceinline__ WaveBlocks<R> wave_blocks(const Lattice3<R>& p, std::uint32_t g0, std::uint32_t back) {
+    WaveBlocks<R> w;
+    w.shift = p.block_shift;
+    w.mask = (1U << p.block_shift) - 1U;
+    w.first = __builtin_amdgcn_readfirstlane(g0 >> p.block_shift) - back;
+    return w;
+}
+
 // Entry i of the storage offset table. The table is read-only while kernels run; reading it
 // through the constant address space lets a uniform read become a scalar load, where a plain
 // global pointer gives a vector load per thread (the kernel's own stores might alias it).
@@ -322,32 +365,19 @@ __device__ __forceinline__ void inlet_west3(typename T::R (&f)[19], const Lattic
 //
 // The loads are split from the rest so that a kernel can issue them before it knows the
 // node's class (the class byte and the 19 populations then travel together instead of one
-// after the other). Any cell index below `cells` gives in-range addresses: the population
-// buffer has guard cells before plane 0 and after plane 18 (at least nx ny + nx + 1 each,
-// lbm_internal.h), so the B-step sources of a non-bulk node, whose values are discarded,
-// stay inside the allocation. Indices are formed from a base moved back by g = nx ny + nx + 1
-// so that they stay unsigned.
+// after the other). Any cell index below `cells` gives in-range addresses: the guard cells
+// around the lattice keep the B-step sources of a non-bulk node, whose values are discarded,
+// inside the allocation.
 template <class T, bool kLocal>
 __device__ __forceinline__ void bulk_load(const Lattice3<typename T::R>& p, std::uint32_t cell,
                                           typename T::R (&v)[19]) {
     using S = typename T::S;
     const S* __restrict__ f = static_cast<const S*>(p.f);
-    const std::uint32_t stride = p.stride;
     const int nx = static_cast<int>(p.nx);
     const int plane = static_cast<int>(p.nx * p.ny);
-    if constexpr (kLocal) {
 #pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            v[i] = decode<T>(p, f[i * stride + cell], i);
-        }
-    } else {
-        const std::uint32_t g = p.nx * p.ny + p.nx + 1U;
-        const S* __restrict__ base = f - g;
-        const std::uint32_t shifted = cell + g;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) {
-            v[i] = decode<T>(p, base[L3::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(shifted) - off3(i, nx, plane))], i);
-        }
+    for (int i = 0; i < 19; ++i) {
+        v[i] = decode<T>(p, kLocal ? f[pop(p, i, cell)] : f[pop(p, L3::opp(i), cell, -off3(i, nx, plane))], i);
     }
 }
 
@@ -357,7 +387,6 @@ __device__ __forceinline__ void bulk_finish(const Lattice3<typename T::R>& p, st
     using S = typename T::S;
     using R = typename T::R;
     S* __restrict__ f = static_cast<S*>(p.f);
-    const std::uint32_t stride = p.stride;
     const int nx = static_cast<int>(p.nx);
     const int plane = static_cast<int>(p.nx * p.ny);
     R sponge = 0;
@@ -371,11 +400,11 @@ __device__ __forceinline__ void bulk_finish(const Lattice3<typename T::R>& p, st
     const std::uint32_t bits = T::dithered ? cell_dither(cell, slot_dither(*p.clock)) : 0U;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
+        const S packed = encode<T>(p, v[i], i, direction_dither(bits, i));
         if constexpr (kLocal) {
-            f[L3::opp(i) * stride + cell] = encode<T>(p, v[i], i, direction_dither(bits, i));
+            f[pop(p, L3::opp(i), cell)] = packed;
         } else {
-            f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] =
-                encode<T>(p, v[i], i, direction_dither(bits, i));
+            f[pop(p, i, cell, off3(i, nx, plane))] = packed;
         }
     }
 }
@@ -413,13 +442,10 @@ template <class R> __device__ __forceinline__ std::uint32_t side_faces(const Lat
 }
 
 // Element of the B-step address of pull i of node `cell` lying on `faces` (0: the plain address of
-// a bulk node), counted from g elements before plane 0 (so that the plain addresses of any cell,
-// which reach up to nx ny + nx + 1 cells outside the planes, stay non-negative for g at least
-// that). Selects only: the result is formed without branches, so a run's loads issue together.
-// Every element fits 32 bits: planes hold at most 2^27 cells.
+// a bulk node). Selects only: the result is formed without branches, so a run's loads issue
+// together.
 template <class R>
-__device__ __forceinline__ std::uint32_t side_element(const Lattice3<R>& p, std::uint32_t cell, std::uint32_t faces, int i,
-                                                      std::uint32_t g) {
+__device__ __forceinline__ std::uint32_t side_element(const Lattice3<R>& p, std::uint32_t cell, std::uint32_t faces, int i) {
     const int cx = L3::cx(i);
     const int cy = L3::cy(i);
     const int cz = L3::cz(i);
@@ -437,8 +463,7 @@ __device__ __forceinline__ std::uint32_t side_element(const Lattice3<R>& p, std:
     const int dz = zcross ? (zslip ? 0 : cz * (static_cast<int>(p.nz) - 1)) : -cz;
     const int plane = pick4(L3::opp(i), L3::opp(L3::mirror_y(i)), L3::opp(L3::mirror_z(i)),
                             L3::opp(L3::mirror_z(L3::mirror_y(i))), my, mz);
-    const int src = static_cast<int>(cell + g) - cx + static_cast<int>(p.nx) * dy + static_cast<int>(p.nx * p.ny) * dz;
-    return static_cast<std::uint32_t>(plane) * p.stride + static_cast<std::uint32_t>(src);
+    return pop(p, plane, cell, -cx + static_cast<int>(p.nx) * dy + static_cast<int>(p.nx * p.ny) * dz);
 }
 
 // The B step of one side node (the side kernel's threads, the fused schedule's lanes).
@@ -450,7 +475,7 @@ template <class T> __device__ __forceinline__ void side_node_b(const Lattice3<ty
     R v[19];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        v[i] = decode<T>(p, f[side_element(p, cell, faces, i, 0U)], i);
+        v[i] = decode<T>(p, f[side_element(p, cell, faces, i)], i);
     }
     R sponge = 0;
     if (p.sponge_strength > R(0)) {
@@ -463,7 +488,7 @@ template <class T> __device__ __forceinline__ void side_node_b(const Lattice3<ty
     const std::uint32_t bits = T::dithered ? cell_dither(cell, slot_dither(*p.clock)) : 0U;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        f[side_element(p, cell, faces, L3::opp(i), 0U)] = encode<T>(p, v[i], i, direction_dither(bits, i));
+        f[side_element(p, cell, faces, L3::opp(i))] = encode<T>(p, v[i], i, direction_dither(bits, i));
     }
 }
 
@@ -475,7 +500,7 @@ __device__ __forceinline__ bool streamed(std::uint8_t kind) {
 // Nodes the bulk kernel updates in B steps: their B-step addresses are the plain ones.
 __device__ __forceinline__ bool plain_b(std::uint8_t kind) { return kind == kBulkNode || kind == kInletNode; }
 
-// V consecutive stored populations of direction i in one plane, as one memory access.
+// V consecutive stored populations of direction i (one block), as one memory access.
 template <class T, int V>
 __device__ __forceinline__ void load_run(const Lattice3<typename T::R>& p, const typename T::S* src, int i,
                                          typename T::R (&out)[V]) {
@@ -499,19 +524,95 @@ __device__ __forceinline__ void store_run(const Lattice3<typename T::R>& p, type
     __builtin_memcpy(dst, tmp, sizeof(tmp));
 }
 
-// The bulk half of a step for the run of V cells starting at c0: each thread takes V
-// consecutive cells and moves each population plane as one V-wide access (19 loads and 19
-// stores per thread whatever V is). A run whose cells are all updated stores whole runs; a
-// mixed run stores its updated cells one by one, because the other cells' addresses belong to
-// nodes the boundary kernel updates.
-// Element `index` of a population plane: a uniform 64-bit plane base plus a 32-bit byte offset,
-// the addressing of the scalar-base memory instructions (one offset register serves every
-// plane of an A step). Byte offsets stay below 2^32: cells and guards are below 2^28.
+// Element `index` of the populations counted from a wave's base: a uniform 64-bit base plus a
+// 32-bit byte offset, the addressing of the scalar-base memory instructions (one offset register
+// serves every direction of an A step).
 template <class S> __device__ __forceinline__ S* element(S* base, std::uint32_t index) {
     using Byte = std::conditional_t<std::is_const_v<S>, const char, char>;
     return reinterpret_cast<S*>(reinterpret_cast<Byte*>(base) + index * static_cast<std::uint32_t>(sizeof(S)));
 }
 
+// A run of V cells displaced by a B-step shift starts at guarded cell g and may cross into the
+// next block (the last V - 1 cells of a block), where its cells are no longer adjacent: such a
+// lane moves them one by one.
+template <class R, int V>
+__device__ __forceinline__ bool crosses_block(const WaveBlocks<R>& w, std::uint32_t g) {
+    return V > 1 && (g & w.mask) > w.mask + 1U - static_cast<std::uint32_t>(V);
+}
+// Whether any run of a wave crosses under the shift d (cells added to each run's start), from
+// uniform values only. A wave's lanes hold consecutive runs starting at multiples of V, 64 V
+// cells aligned to 64 V within one block (`group`: their position in the block rounded down).
+// A run starting at g crosses iff (g + d) mod 2^k > 2^k - V; with g a multiple of V that needs d
+// not a multiple of V and then fixes g mod 2^k to one value, 2^k - V - (d rounded down to a
+// multiple of V).
+template <class R, int V>
+__device__ __forceinline__ bool wave_crosses(const WaveBlocks<R>& w, std::uint32_t group, int d) {
+    if (V == 1 || (d & (V - 1)) == 0) {
+        return false;
+    }
+    const std::uint32_t start = static_cast<std::uint32_t>(-V - (d & ~(V - 1))) & w.mask;
+    return (start & ~(64U * V - 1U)) == group;
+}
+
+// The B-step loads of a run of V cells, direction by direction: every direction's combined
+// V-wide load (a crossing run's reads past its block's direction, inside the buffer: the guard
+// holds V cells more than any shift), then, only in a wave that has a crossing run, the cells
+// of the crossing runs one by one.
+template <class T, int V>
+__device__ __forceinline__ void load_shifted_all(const WaveBlocks<typename T::R>& w, const typename T::S* base,
+                                                 std::uint32_t g0, std::uint32_t group, int nx, int plane,
+                                                 typename T::S (&raw)[19][V]) {
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const std::uint32_t g = static_cast<std::uint32_t>(static_cast<int>(g0) - off3(i, nx, plane));
+        __builtin_memcpy(raw[i], element(base, w.offset(g, static_cast<std::uint32_t>(L3::opp(i)))), sizeof(raw[i]));
+    }
+    if constexpr (V > 1) {
+        bool fix = false;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            fix = fix || wave_crosses<typename T::R, V>(w, group, -off3(i, nx, plane));
+        }
+        if (fix) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                const std::uint32_t g = static_cast<std::uint32_t>(static_cast<int>(g0) - off3(i, nx, plane));
+                if (crosses_block<typename T::R, V>(w, g)) {
+#pragma unroll
+                    for (int k = 1; k < V; ++k) {
+                        raw[i][k] = *element(base, w.offset(g + static_cast<std::uint32_t>(k),
+                                                            static_cast<std::uint32_t>(L3::opp(i))));
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <class T, int V>
+__device__ __forceinline__ void store_shifted(const Lattice3<typename T::R>& p, const WaveBlocks<typename T::R>& w,
+                                              typename T::S* base, std::uint32_t g, bool careful, int i,
+                                              const typename T::R (&in)[V], const std::uint32_t (&bits)[V]) {
+    typename T::S tmp[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        tmp[k] = encode<T>(p, in[k], i, direction_dither(bits[k], i));
+    }
+    if (careful && crosses_block<typename T::R, V>(w, g)) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            *element(base, w.offset(g + static_cast<std::uint32_t>(k), static_cast<std::uint32_t>(i))) = tmp[k];
+        }
+    } else {
+        __builtin_memcpy(element(base, w.offset(g, static_cast<std::uint32_t>(i))), tmp, sizeof(tmp));
+    }
+}
+
+// The bulk half of a step for the run of V cells starting at c0: each thread takes V
+// consecutive cells and moves each population of the run as one V-wide access (19 loads and 19
+// stores per thread whatever V is). A run whose cells are all updated stores whole runs; a
+// mixed run stores its updated cells one by one, because the other cells' addresses belong to
+// nodes the boundary kernel updates. Lanes of a wave take increasing c0.
 template <class T, bool kLocal, int V>
 __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::uint32_t c0) {
     using S = typename T::S;
@@ -519,7 +620,7 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
     if (c0 >= p.cells) {
         return;
     }
-    // The class array is padded with solid (0) entries up to the plane stride. The A step
+    // The class array is padded with solid (0) entries beyond the last cell. The A step
     // updates bulk, side and inlet nodes (a side node's A step is a bulk node's); the B step bulk
     // and inlet nodes, side nodes having a kernel of their own then (side_node_b).
     std::uint8_t kind[V];
@@ -536,26 +637,41 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
         return;
     }
     S* __restrict__ f = static_cast<S*>(p.f);
-    const std::uint32_t stride = p.stride;
     const int nx = static_cast<int>(p.nx);
     const int plane = static_cast<int>(p.nx * p.ny);
-    // B-step sources of a run reach nx ny + nx + 1 cells back (and V - 1 + that forward): the
-    // guard cells around the planes keep them in the allocation.
-    const std::uint32_t g = p.nx * p.ny + p.nx + 1U;
+    const std::uint32_t g0 = c0 + p.guard;
+    // B-step sources and targets of a run reach nx ny + nx + 1 cells either way (and V - 1
+    // further): the guard blocks behind the wave's first block keep their offsets positive.
+    const WaveBlocks<R> w = wave_blocks(p, g0, kLocal ? 0U : p.guard >> p.block_shift);
+    S* __restrict__ base = f + (static_cast<std::size_t>(w.first) * 19U << w.shift);
+    const std::uint32_t own = w.offset0(g0);
+    // the wave's runs lie in one 64 V-cell group of a block (blocks of threads are whole waves)
+    const std::uint32_t group = __builtin_amdgcn_readfirstlane(g0) & w.mask & ~(64U * V - 1U);
     R v[V][19];
+    if constexpr (kLocal || V == 1) {
 #pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        R run[V];
-        if constexpr (kLocal) {
-            load_run<T, V>(p, element(f + static_cast<std::size_t>(i) * stride, c0), i, run);
-        } else {
-            load_run<T, V>(p, element(f + static_cast<std::size_t>(L3::opp(i)) * stride - g,
-                                   static_cast<std::uint32_t>(static_cast<int>(c0 + g) - off3(i, nx, plane))),
-                           i, run);
+        for (int i = 0; i < 19; ++i) {
+            R run[V];
+            if constexpr (kLocal) {
+                load_run<T, V>(p, element(base + (static_cast<std::size_t>(i) << w.shift), own), i, run);
+            } else {
+                const std::uint32_t g = static_cast<std::uint32_t>(static_cast<int>(g0) - off3(i, nx, plane));
+                load_run<T, V>(p, element(base, w.offset(g, static_cast<std::uint32_t>(L3::opp(i)))), i, run);
+            }
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                v[c][i] = run[c];
+            }
         }
+    } else {
+        S raw[19][V];
+        load_shifted_all<T, V>(w, base, g0, group, nx, plane, raw);
 #pragma unroll
-        for (int c = 0; c < V; ++c) {
-            v[c][i] = run[c];
+        for (int i = 0; i < 19; ++i) {
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                v[c][i] = decode<T>(p, raw[i][c], i);
+            }
         }
     }
     const auto updated = [&](int c) { return kLocal ? streamed(kind[c]) : plain_b(kind[c]); };
@@ -587,6 +703,13 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
         }
     }
     if (all) {
+        bool careful = false;  // a run of the wave crosses a block in some direction
+        if constexpr (!kLocal) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                careful = careful || wave_crosses<R, V>(w, group, off3(i, nx, plane));
+            }
+        }
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
             R run[V];
@@ -595,25 +718,25 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
                 run[c] = v[c][i];
             }
             if constexpr (kLocal) {
-                store_run<T, V>(p, element(f + static_cast<std::size_t>(L3::opp(i)) * stride, c0), run, bits, i);
+                store_run<T, V>(p, element(base + (static_cast<std::size_t>(L3::opp(i)) << w.shift), own), run, bits, i);
             } else {
-                store_run<T, V>(p, element(f + static_cast<std::size_t>(i) * stride,
-                                        static_cast<std::uint32_t>(static_cast<int>(c0) + off3(i, nx, plane))),
-                                run, bits, i);
+                store_shifted<T, V>(p, w, base, static_cast<std::uint32_t>(static_cast<int>(g0) + off3(i, nx, plane)),
+                                    careful, i, run, bits);
             }
         }
     } else {
 #pragma unroll
         for (int c = 0; c < V; ++c) {
             if (updated(c)) {
-                const std::uint32_t cell = c0 + c;
+                const std::uint32_t g = g0 + static_cast<std::uint32_t>(c);
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
+                    const S packed = encode<T>(p, v[c][i], i, direction_dither(bits[c], i));
                     if constexpr (kLocal) {
-                        f[L3::opp(i) * stride + cell] = encode<T>(p, v[c][i], i, direction_dither(bits[c], i));
+                        *element(base, w.offset(g, static_cast<std::uint32_t>(L3::opp(i)))) = packed;
                     } else {
-                        f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane))] =
-                            encode<T>(p, v[c][i], i, direction_dither(bits[c], i));
+                        *element(base, w.offset(static_cast<std::uint32_t>(static_cast<int>(g) + off3(i, nx, plane)),
+                                                static_cast<std::uint32_t>(i))) = packed;
                     }
                 }
             }
@@ -662,7 +785,6 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
     using S = typename T::S;
     using R = typename T::R;
     S* __restrict__ f = static_cast<S*>(p.f);
-    const std::uint32_t stride = p.stride;
     const std::uint32_t bs = p.bstride;
     const int nx = static_cast<int>(p.nx);
     const int plane = static_cast<int>(p.nx * p.ny);
@@ -708,12 +830,12 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
         if constexpr (kLocal) {
             // A step: pulls are the node's own addresses, direction i or opposite(i) (a source
             // beyond a slip side stored its mirrored output here too)
-            fv[i] = (pull || far) ? f[(far ? L3::opp(i) : i) * stride + cell] : S(0);
+            fv[i] = (pull || far) ? f[pop(p, far ? L3::opp(i) : i, cell)] : S(0);
         } else {
             // B step: A[opp(d)][s] for the source node s and its direction d after the mirrors
             const std::uint32_t src = far ? static_cast<std::uint32_t>(static_cast<int>(cell) + off3(i, nx, plane)) : payload;
             const std::uint32_t at = far ? static_cast<std::uint32_t>(i) : mirrored_opp(i, code[i]);
-            fv[i] = (pull || far) ? f[at * stride + src] : S(0);
+            fv[i] = (pull || far) ? f[pop(p, static_cast<int>(at), src)] : S(0);
         }
         const bool other = kind == kOutlet || kind == kStash;
         sv[i] = other ? st[mirrored(i, code[i]) * bs + payload] : R(0);
@@ -786,7 +908,7 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
             continue;
         }
         if constexpr (kLocal) {
-            f[L3::opp(i) * stride + cell] = packed;
+            f[pop(p, L3::opp(i), cell)] = packed;
         } else {
             // The reader of output i and the direction it pulls: x + c_i through the sides
             // (periodic: wrapped; slip: the row kept and the direction mirrored, a reader that
@@ -830,7 +952,7 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
             }
             if (inside) {
                 const int d = pick4(i, L3::mirror_y(i), L3::mirror_z(i), L3::mirror_z(L3::mirror_y(i)), my, mz);
-                f[static_cast<std::uint32_t>(d) * stride + static_cast<std::uint32_t>(tx + nx * ty + plane * tz)] = packed;
+                f[pop(p, d, static_cast<std::uint32_t>(tx + nx * ty + plane * tz))] = packed;
             }
         }
     }

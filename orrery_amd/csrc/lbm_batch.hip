@@ -168,9 +168,9 @@ __global__ __launch_bounds__(256) void batch_force(const Lattice3<typename T::R>
             if (phase == 0) {
                 const int off = d3::off3(j, static_cast<int>(p.nx), static_cast<int>(p.nx * p.ny));
                 farther = static_cast<double>(
-                    d3::decode<T>(p, f[j * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off)], o));
+                    d3::decode<T>(p, f[d3::pop(p, j, cell, off)], o));
             } else {
-                farther = static_cast<double>(d3::decode<T>(p, f[o * p.stride + cell], o));
+                farther = static_cast<double>(d3::decode<T>(p, f[d3::pop(p, o, cell)], o));
             }
             if constexpr (T::shifted) {
                 farther += L3::w(o);

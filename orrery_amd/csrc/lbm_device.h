@@ -1,8 +1,9 @@
 // Device-side definitions shared by the D3Q19 and D2Q9 kernels: population storage formats,
 // the in-place (A-A) addressing, launch-shape helpers and the lattice descriptors.
 //
-// In-place A-A scheme (Bailey, Myre, Walsh, Lilja and Pickett 2009). One array holds 19 (9)
-// planes of `stride` populations. Let n be the number of completed steps.
+// In-place A-A scheme (Bailey, Myre, Walsh, Lilja and Pickett 2009). One array holds the 19 (9)
+// populations of every cell, A[i][x] below (the D3Q19 layout in blocks of cells, Lattice3; the
+// D2Q9 one in planes of `stride`, Lattice2). Let n be the number of completed steps.
 //  - n even ("post-A" layout): the post-collision population f*_i(x) sits at A[opp(i)][x].
 //  - n odd  ("post-B" layout): f*_i(x) sits at A[i][x + c_i] (in-domain fluid targets only).
 // Step n even (B step): node x reads its pull values f*_i(x - c_i) at A[opp(i)][x - c_i] and
@@ -183,7 +184,13 @@ __host__ __device__ constexpr std::uint32_t make_code(int kind, bool my, bool mz
 
 // ---- lattice descriptor (kernel argument) ---------------------------------------------------
 template <typename R> struct Lattice3 {
-    void* f;                  // S[19][stride]
+    // Populations in blocks of 2^block_shift consecutive cells, each block [19][2^block_shift]:
+    // A[i][c] is element (c' >> k) 19 2^k + i 2^k + (c' mod 2^k), c' = c + guard, k = block_shift.
+    // `guard` (whole blocks) cells before the first cell and at least as many after the last
+    // keep the plain addresses of every cell's neighbours (nx ny + nx + 1 cells away, and a
+    // run's V - 1 more) inside the buffer. A wave's 19 accesses of an A step then fall in one
+    // span of 19 2^k elements instead of 19 planes across the buffer.
+    void* f;
     const std::uint8_t* cls;  // node class per cell (NodeClass)
     const std::uint32_t* bcell;  // boundary node -> cell [nb]
     const std::uint32_t* bcode;  // [19][bstride]
@@ -194,7 +201,9 @@ template <typename R> struct Lattice3 {
     const std::uint32_t* tile_slot;  // [cells / 64 + 1] boundary nodes before each 64-cell run
     const std::uint32_t* scell;      // side node -> cell [ns] (cell order): the side kernel's list
     const std::uint32_t* clock;      // steps since initialisation (the 16-bit formats' dither epoch)
-    std::uint32_t nx, ny, nz, cells, stride, nb, bstride, ns;
+    std::uint32_t nx, ny, nz, cells, nb, bstride, ns;
+    std::uint32_t block_shift;  // k: 2^k cells per block
+    std::uint32_t guard;        // guard cells before the first cell (a multiple of 2^k)
     FastDiv div_nx, div_plane;
     int x_mode;       // 0 periodic, 1 inflow/outflow
     int convective;   // outlet blends with history
