@@ -1,11 +1,11 @@
 """CPU emulation of the in-place (A-A) index map of the D3Q19 bulk kernel, with labels in place of
 populations.
 
-The bulk kernel (orrery_amd/csrc/lbm3d_step.h, bulk_run) addresses element e of plane k of the
-single population array as below, with n the number of completed steps:
+The bulk kernel (orrery_amd/csrc/lbm3d_step.h, bulk_run) addresses population A[k][e] (plane k,
+cell e) of the single population array as below, with n the number of completed steps:
 - n odd, A step: node x loads plane i at x and stores its output i to plane opp(i) at x;
-- n even, B step: node x loads plane opp(i) at x - c_i (through a base moved back by the guard
-  g = nx ny + nx + 1) and stores output i to plane i at x + c_i.
+- n even, B step: node x loads plane opp(i) at x - c_i (cells up to nx ny + nx + 1 outside the
+  lattice: the guard cells) and stores output i to plane i at x + c_i.
 The readback kernels report f*_i(x) from plane opp(i) at x after an even count and from plane
 i at x + c_i after an odd one.
 
@@ -14,8 +14,9 @@ steps of a lattice whose interior nodes are bulk nodes: every node reads exactly
 it writes (the in-place property), no address is written twice in a step, a bulk node's pull i
 is the label its upstream bulk node x - c_i wrote one step before (the streaming), and the
 readback map finds every label. A planted mistake in the map (the B step's store without its
-shift) breaks the streaming check. The second half does the same for side nodes (face nodes
-streamed in place through slip and periodic sides) next to the listed boundary nodes.
+shift) breaks the streaming check. The second part does the same for side nodes (face nodes
+streamed in place through slip and periodic sides) next to the listed boundary nodes; the third
+checks how the buffer stores A[k][e] (blocks of cells) and the bulk kernel's run accesses to it.
 """
 import numpy as np
 import pytest
@@ -294,3 +295,125 @@ def test_side_nodes_stream_in_place_through_slip_and_periodic_sides(dims, sides)
 def test_side_nodes_without_their_mirrored_addresses_are_caught():
     failures = run_sides(SideLattice(6, 5, 4, (SLIP,) * 4, mirror_side_loads=False), 3)
     assert any(f[0] == "stream" for f in failures)
+
+
+# ---- the blocked layout of the population buffer ---------------------------------------------------
+# The (plane, element) addresses above are stored in blocks of 2^k cells (Lattice3::f,
+# orrery_amd/csrc/lbm_device.h): A[dir][c] is element (g >> k) 19 2^k + dir 2^k + (g mod 2^k),
+# g = c + guard (guarded_element, lbm3d_step.h). The bulk kernel moves runs of V consecutive cells
+# with one access per direction; in the B step a run displaced by an x shift may cross into the
+# next block, where its cells are no longer adjacent, and such runs are moved cell by cell
+# (load_shifted_all, store_shifted), which a wave decides from uniform values (wave_crosses).
+# The emulation checks, on layouts with small blocks (many crossings): the element map is a
+# bijection onto the buffer; every access a B-step run makes, the combined access included,
+# stays in the buffer; after the fix-ups each cell's load and store element is the per-cell one;
+# the per-wave test agrees with a lane-by-lane scan; without the fix-ups the check fails.
+
+
+def padded(n, m):
+    return (n + m - 1) // m * m
+
+
+class Blocked:
+    """population_layout (lbm_internal.h) with a given block shift."""
+
+    def __init__(self, nx, ny, nz, shift):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.cells = nx * ny * nz
+        self.shift = shift
+        self.block = 1 << shift
+        self.mask = self.block - 1
+        self.guard = padded(nx * ny + nx + 1 + 4, self.block)
+        self.blocks = (self.cells + 2 * self.guard + self.block - 1) // self.block
+        self.elements = 19 * self.block * self.blocks
+
+    def off(self, i):
+        return CX[i] + self.nx * CY[i] + self.nx * self.ny * CZ[i]
+
+    def element(self, g, d):
+        return (((g >> self.shift) * 19 + d) << self.shift) | (g & self.mask)
+
+    def crosses(self, g, v):
+        return v > 1 and (g & self.mask) > self.mask + 1 - v
+
+    def wave_crosses(self, group, dshift, v):
+        if v == 1 or dshift & (v - 1) == 0:
+            return False
+        start = (-v - (dshift & ~(v - 1))) & self.mask
+        return start & ~(64 * v - 1) == group
+
+
+def b_step_run(lat, c0, v, fix=True):
+    """Elements a run of v cells starting at c0 loads and stores per direction in the B step, as
+    the bulk kernel forms them (combined access from the run's first element, then the crossing
+    runs' cells one by one), and the elements its combined accesses touch."""
+    g0 = c0 + lat.guard
+    loads, stores, touched = [], [], []
+    for i in range(19):
+        g = g0 - lat.off(i)
+        e = lat.element(g, OPP[i])
+        run = [e + k for k in range(v)]
+        touched += run
+        if fix and lat.crosses(g, v):
+            run = [run[0]] + [lat.element(g + k, OPP[i]) for k in range(1, v)]
+            touched += run
+        loads.append(run)
+        g = g0 + lat.off(i)
+        if fix and lat.crosses(g, v):
+            run = [lat.element(g + k, i) for k in range(v)]
+        else:
+            run = [lat.element(g, i) + k for k in range(v)]
+        touched += run
+        stores.append(run)
+    return loads, stores, touched
+
+
+@pytest.mark.parametrize("dims,shift", [((12, 6, 5), 8), ((20, 7, 6), 8), ((16, 16, 6), 9)])
+def test_blocked_layout_is_a_bijection_onto_the_buffer(dims, shift):
+    lat = Blocked(*dims, shift)
+    seen = np.zeros(lat.elements, dtype=bool)
+    for g in range(lat.blocks * lat.block):
+        for d in range(19):
+            e = lat.element(g, d)
+            assert 0 <= e < lat.elements and not seen[e]
+            seen[e] = True
+    assert seen.all()
+
+
+@pytest.mark.parametrize("dims,shift", [((12, 6, 5), 8), ((20, 7, 6), 8), ((17, 9, 5), 8), ((16, 16, 6), 9)])
+@pytest.mark.parametrize("v", [2, 4])
+def test_blocked_runs_address_their_cells_in_the_b_step(dims, shift, v):
+    lat = Blocked(*dims, shift)
+    crossing = 0
+    for c0 in range(0, padded(lat.cells, v), v):
+        loads, stores, touched = b_step_run(lat, c0, v)
+        assert all(0 <= e < lat.elements for e in touched)
+        for i in range(19):
+            truth_load = [lat.element(c0 + k + lat.guard - lat.off(i), OPP[i]) for k in range(v)]
+            truth_store = [lat.element(c0 + k + lat.guard + lat.off(i), i) for k in range(v)]
+            assert loads[i] == truth_load
+            assert stores[i] == truth_store
+            crossing += lat.crosses(c0 + lat.guard - lat.off(i), v)
+    assert crossing > 0  # the case the fix-ups exist for does occur
+
+
+@pytest.mark.parametrize("v", [2, 4])
+def test_the_per_wave_crossing_test_matches_a_lane_scan(v):
+    for shift in (8, 9, 11):
+        lat = Blocked(4, 4, 4, shift)
+        shifts = [s for s in range(-3 * lat.block - 5, 3 * lat.block + 5, 3)] + [1, -1, 2, -2, 3, -3]
+        for group in range(0, lat.block, 64 * v):
+            for d in shifts:
+                scan = any(((group + v * lane + d) & lat.mask) > lat.block - v for lane in range(64))
+                assert lat.wave_crosses(group, d, v) == scan, (shift, group, d)
+
+
+def test_blocked_runs_without_the_fix_ups_are_caught():
+    lat = Blocked(12, 6, 5, 8)
+    wrong = 0
+    for c0 in range(0, lat.cells, 2):
+        loads, _, _ = b_step_run(lat, c0, 2, fix=False)
+        for i in range(19):
+            truth = [lat.element(c0 + k + lat.guard - lat.off(i), OPP[i]) for k in range(2)]
+            wrong += loads[i] != truth
+    assert wrong > 0
