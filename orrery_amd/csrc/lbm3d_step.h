@@ -633,7 +633,13 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
         any = any || mine;
         all = all && mine;
     }
-    if (!any) {
+    // A run's population loads either wait for its classes (a run without an update is skipped)
+    // or are issued with them (the run then loads and stores nothing it does not update): the
+    // second measured faster for the A step of the 4- and 8-byte formats and the B step of the
+    // 16-bit ones (512x256x256: f32 A 915 -> 902 us, f16s B 565 -> 525 us), the first for the
+    // other two (f32 B 975 against 1006 us under the tuned occupancy caps, f16s A 452 against 466).
+    constexpr bool kLoadsWithClasses = sizeof(S) == 2 ? !kLocal : kLocal;
+    if (!kLoadsWithClasses && !any) {
         return;
     }
     S* __restrict__ f = static_cast<S*>(p.f);
