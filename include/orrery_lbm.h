@@ -156,7 +156,7 @@ int lbm3d_device_bytes(const lbm3d* h, size_t* bytes);
 /* Blocks until the handle's stream has drained (the reference's finish()). */
 int lbm3d_synchronize(lbm3d* h);
 /* Launch shape of the bulk kernel: block size (64..256, multiple of 64) and cells per thread
- * (1, 2 or 4 consecutive cells, each population plane moved as one access). 0 keeps the
+ * (1, 2 or 4 consecutive cells, each direction of the run moved as one access). 0 keeps the
  * current value. Used by the autotuner. */
 int lbm3d_set_launch(lbm3d* h, int32_t block, int32_t cells_per_thread);
 /* Cells per thread of the bulk kernel for the A step (odd step count) and the B step (even)
@@ -182,7 +182,7 @@ int lbm3d_get_launch(const lbm3d* h, int32_t* schedule, int32_t* block, int32_t*
 /* Cap on the workgroups of the bulk kernel that share a compute unit, for the A step (odd step
  * counts) and the B step (even), 1 to 32, 0 for none (the default): each workgroup reserves LDS
  * it does not use, so that no more than that many fit (with 256-thread workgroups, the cap is
- * waves per SIMD). Fewer resident waves stream the 19 population planes with fewer DRAM
+ * waves per SIMD). Fewer resident waves stream the 19 population directions with fewer DRAM
  * conflicts; results do not change. Refused when the device's LDS cannot express the cap. No
  * reference counterpart; the autotuner sets it. */
 int lbm3d_set_occupancy(lbm3d* h, int32_t groups_a, int32_t groups_b);
