@@ -865,10 +865,15 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
     }
 
     R v[19];
+    // The convective outlet's new history values are written with the other stores after the
+    // collision: a store among the rules would make every later wait for a load wait for it too
+    // (one counter covers loads and stores).
+    R hist_new[19];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const int kind = static_cast<int>(code[i] & 7U);
         const std::uint32_t payload = code[i] >> kCodeShift;
+        hist_new[i] = R(0);
         if (kind == kNode) {
             v[i] = decode_mirrored<T>(p, fv[i], i, code[i]);
         } else if (kind == kStash) {
@@ -894,7 +899,7 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
             R up = sv[i];
             if (p.convective) {
                 up = (hist[i] + p.outlet_speed * up) / (R(1) + p.outlet_speed);
-                p.hist[L3::outlet_slot(i) * bs + b] = up;
+                hist_new[i] = up;
             }
             v[i] = up;
         }
@@ -906,6 +911,12 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
     collide3<T>(v, p, sponge);
     const std::uint32_t bits = T::dithered ? cell_dither(cell, slot_dither(*p.clock)) : 0U;
     const std::uint32_t wmask = p.bwrite[b];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        if (L3::cx(i) < 0 && p.convective && static_cast<int>(code[i] & 7U) == kOutlet) {
+            p.hist[L3::outlet_slot(i) * bs + b] = hist_new[i];
+        }
+    }
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const S packed = encode<T>(p, v[i], i, direction_dither(bits, i));
