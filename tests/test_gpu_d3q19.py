@@ -258,7 +258,10 @@ def test_schedules_and_launch_shapes_are_bitwise_identical(storage, x_periodic):
     occupancy cap), and the two kernels on two streams give the same state bit for bit. 70 x 21 x 19 cells is not a
     multiple of 64, so the last wave of the fused kernel is partial; with 4 cells per thread
     some runs hold the x = nx - 1 node of one row and the x = 0 node of the next. With periodic
-    x, the x-face nodes pull across the row ends."""
+    x, the x-face nodes pull across the row ends. The populations are stored in blocks of 2048
+    cells (the plane holds 1470), whose edges fall at varying x: the B-step runs of 2 and 4 cells
+    that an x shift carries across a block edge, moved cell by cell, meet the fused kernel's
+    per-cell addresses."""
     c, mask = sphere_channel(70, 21, 19, 7.0, 2.0, 0.05, 60.0)
     c.smagorinsky = Smagorinsky(True, 0.17)
     c.solid_wall = SolidWall.interpolated
