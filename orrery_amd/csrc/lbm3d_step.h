@@ -612,7 +612,10 @@ __device__ __forceinline__ void store_shifted(const Lattice3<typename T::R>& p, 
 // consecutive cells and moves each population of the run as one V-wide access (19 loads and 19
 // stores per thread whatever V is). A run whose cells are all updated stores whole runs; a
 // mixed run stores its updated cells one by one, because the other cells' addresses belong to
-// nodes the boundary kernel updates. Lanes of a wave take increasing c0.
+// nodes the boundary kernel updates. The lanes of a wave take increasing c0, 64 V consecutive
+// cells aligned to 64 V (the callers' blocks of threads are whole waves: multiples of 64, which
+// lbm3d_set_launch and lbm_batch_set_launch enforce); the wave addressing and the per-wave test
+// of block-edge runs rely on it.
 template <class T, bool kLocal, int V>
 __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::uint32_t c0) {
     using S = typename T::S;
