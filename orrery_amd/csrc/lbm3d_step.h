@@ -554,6 +554,22 @@ __device__ __forceinline__ bool wave_crosses(const WaveBlocks<R>& w, std::uint32
     return (start & ~(64U * V - 1U)) == group;
 }
 
+// Whether the wave's 64 V cells (`group`: their position in the block), displaced by the shift
+// d, all stay inside one block, from uniform values only.
+template <class R, int V>
+__device__ __forceinline__ bool shift_inside(const WaveBlocks<R>& w, std::uint32_t group, int d) {
+    return ((group + static_cast<std::uint32_t>(d)) & w.mask) <= w.mask + 1U - 64U * V;
+}
+// For such a shift, the element offset of A[dir][g + d] less the lane's own offset0(g), the
+// same for every lane of the wave: the shift moves q = floor((group + d) / 2^k) blocks (q < 0:
+// back), that is 19 q block lengths of the layout, of which q 2^k cells are covered by d itself.
+// Unsigned arithmetic: the sum with offset0(g) is the offset modulo 2^32, which is exact.
+template <class R>
+__device__ __forceinline__ std::uint32_t shift_term(const WaveBlocks<R>& w, std::uint32_t group, int d, int dir) {
+    const int q = (static_cast<int>(group) + d) >> w.shift;
+    return (static_cast<std::uint32_t>(q * 18 + dir) << w.shift) + static_cast<std::uint32_t>(d);
+}
+
 // The B-step loads of a run of V cells, direction by direction: every direction's combined
 // V-wide load (a crossing run's reads past its block's direction, inside the buffer: the guard
 // holds V cells more than any shift), then, only in a wave that has a crossing run, the cells
@@ -656,21 +672,38 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
     const std::uint32_t own = w.offset0(g0);
     // the wave's runs lie in one 64 V-cell group of a block (blocks of threads are whole waves)
     const std::uint32_t group = __builtin_amdgcn_readfirstlane(g0) & w.mask & ~(64U * V - 1U);
+    // B step: where the wave's cells stay inside one block under every direction's shift (the
+    // load shifts -off3(i) are the store shifts off3(opp(i))), each lane's offsets are its own
+    // offset plus uniform terms; other waves form them lane by lane.
+    bool inside = true;
+    if constexpr (!kLocal) {
+#pragma unroll
+        for (int i = 1; i < 19; ++i) {
+            inside = inside && shift_inside<R, V>(w, group, -off3(i, nx, plane));
+        }
+    }
     R v[V][19];
-    if constexpr (kLocal || V == 1) {
+    if (kLocal || inside) {
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
             R run[V];
             if constexpr (kLocal) {
                 load_run<T, V>(p, element(base + (static_cast<std::size_t>(i) << w.shift), own), i, run);
             } else {
-                const std::uint32_t g = static_cast<std::uint32_t>(static_cast<int>(g0) - off3(i, nx, plane));
-                load_run<T, V>(p, element(base, w.offset(g, static_cast<std::uint32_t>(L3::opp(i)))), i, run);
+                load_run<T, V>(p, element(base, own + shift_term(w, group, -off3(i, nx, plane), L3::opp(i))), i, run);
             }
 #pragma unroll
             for (int c = 0; c < V; ++c) {
                 v[c][i] = run[c];
             }
+        }
+    } else if constexpr (V == 1) {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            R run[V];
+            const std::uint32_t g = static_cast<std::uint32_t>(static_cast<int>(g0) - off3(i, nx, plane));
+            load_run<T, V>(p, element(base, w.offset(g, static_cast<std::uint32_t>(L3::opp(i)))), i, run);
+            v[0][i] = run[0];
         }
     } else {
         S raw[19][V];
@@ -711,14 +744,7 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
             bits[c] = T::dithered ? cell_dither(c0 + static_cast<std::uint32_t>(c), epoch) : 0U;
         }
     }
-    if (all) {
-        bool careful = false;  // a run of the wave crosses a block in some direction
-        if constexpr (!kLocal) {
-#pragma unroll
-            for (int i = 0; i < 19; ++i) {
-                careful = careful || wave_crosses<R, V>(w, group, off3(i, nx, plane));
-            }
-        }
+    if (all && (kLocal || inside)) {
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
             R run[V];
@@ -729,9 +755,24 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
             if constexpr (kLocal) {
                 store_run<T, V>(p, element(base + (static_cast<std::size_t>(L3::opp(i)) << w.shift), own), run, bits, i);
             } else {
-                store_shifted<T, V>(p, w, base, static_cast<std::uint32_t>(static_cast<int>(g0) + off3(i, nx, plane)),
-                                    careful, i, run, bits);
+                store_run<T, V>(p, element(base, own + shift_term(w, group, off3(i, nx, plane), i)), run, bits, i);
             }
+        }
+    } else if (all) {
+        bool careful = false;  // a run of the wave crosses a block in some direction
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            careful = careful || wave_crosses<R, V>(w, group, off3(i, nx, plane));
+        }
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            R run[V];
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                run[c] = v[c][i];
+            }
+            store_shifted<T, V>(p, w, base, static_cast<std::uint32_t>(static_cast<int>(g0) + off3(i, nx, plane)),
+                                careful, i, run, bits);
         }
     } else {
 #pragma unroll

@@ -308,6 +308,10 @@ def test_side_nodes_without_their_mirrored_addresses_are_caught():
 # bijection onto the buffer; every access a B-step run makes, the combined access included,
 # stays in the buffer; after the fix-ups each cell's load and store element is the per-cell one;
 # the per-wave test agrees with a lane-by-lane scan; without the fix-ups the check fails.
+# A wave whose 64 V cells stay inside one block under every direction's shift (shift_inside)
+# forms its B-step elements as each lane's own element of direction 0 plus a uniform term
+# (shift_term); the emulation checks those elements against the per-cell ones, the uniform test
+# against a lane scan, and that a planted mistake in the term is caught.
 
 
 def padded(n, m):
@@ -341,6 +345,13 @@ class Blocked:
             return False
         start = (-v - (dshift & ~(v - 1))) & self.mask
         return start & ~(64 * v - 1) == group
+
+    def shift_inside(self, group, d, v):
+        return ((group + d) & self.mask) <= self.mask + 1 - 64 * v
+
+    def shift_term(self, group, d, direction, per_block=18):
+        q = (group + d) >> self.shift  # floor, also for a shift back
+        return (((q * per_block + direction) << self.shift) + d) % 2**32
 
 
 def b_step_run(lat, c0, v, fix=True):
@@ -417,3 +428,46 @@ def test_blocked_runs_without_the_fix_ups_are_caught():
             truth = [lat.element(c0 + k + lat.guard - lat.off(i), OPP[i]) for k in range(2)]
             wrong += loads[i] != truth
     assert wrong > 0
+
+
+def uniform_term_mismatches(lat, v, per_block=18):
+    """Waves inside a block under every shift, and the elements their uniform terms get wrong."""
+    inside_waves, wrong = 0, 0
+    for c0 in range(0, padded(lat.cells, 64 * v), 64 * v):
+        g0 = c0 + lat.guard
+        group = g0 & lat.mask & ~(64 * v - 1)
+        inside = all(lat.shift_inside(group, -lat.off(i), v) for i in range(1, 19))
+        # the load shifts of all directions are their store shifts, and the uniform test is a lane scan
+        assert inside == all(lat.shift_inside(group, lat.off(i), v) for i in range(1, 19))
+        assert inside == all(((g0 + k - lat.off(i)) >> lat.shift) == ((g0 - lat.off(i)) >> lat.shift)
+                             for i in range(19) for k in range(64 * v))
+        if not inside:
+            continue
+        inside_waves += 1
+        for k in range(64 * v):
+            g = g0 + k
+            own = lat.element(g, 0)
+            for i in range(19):
+                load = (own + lat.shift_term(group, -lat.off(i), OPP[i], per_block)) % 2**32
+                store = (own + lat.shift_term(group, lat.off(i), i, per_block)) % 2**32
+                wrong += load != lat.element(g - lat.off(i), OPP[i])
+                wrong += store != lat.element(g + lat.off(i), i)
+    return inside_waves, wrong
+
+
+# shapes whose inside waves include shifts that move whole blocks (q != 0) at every v, and two
+# where they do not
+@pytest.mark.parametrize("dims,shift", [((20, 30, 6), 11), ((40, 30, 6), 10), ((24, 40, 5), 10), ((70, 9, 5), 11),
+                                        ((12, 6, 5), 10), ((17, 9, 5), 10)])
+@pytest.mark.parametrize("v", [1, 2, 4])
+def test_waves_inside_a_block_address_by_a_uniform_term(dims, shift, v):
+    lat = Blocked(*dims, shift)
+    inside_waves, wrong = uniform_term_mismatches(lat, v)
+    assert inside_waves > 0 and wrong == 0
+
+
+def test_a_planted_mistake_in_the_uniform_term_is_caught():
+    # 600-cell planes in blocks of 512 cells: the z shifts move whole blocks (q != 0)
+    lat = Blocked(20, 30, 6, 9)
+    inside_waves, wrong = uniform_term_mismatches(lat, 1, per_block=19)
+    assert inside_waves > 0 and wrong > 0
