@@ -400,7 +400,7 @@ __device__ __forceinline__ void bulk_finish(const Lattice3<typename T::R>& p, st
     const std::uint32_t bits = T::dithered ? cell_dither(cell, slot_dither(*p.clock)) : 0U;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        const S packed = encode<T>(p, v[i], i, direction_dither(bits, i));
+        const S packed = encode<T>(p, v[i], i, direction_dither<T>(bits, i));
         if constexpr (kLocal) {
             f[pop(p, L3::opp(i), cell)] = packed;
         } else {
@@ -488,7 +488,7 @@ template <class T> __device__ __forceinline__ void side_node_b(const Lattice3<ty
     const std::uint32_t bits = T::dithered ? cell_dither(cell, slot_dither(*p.clock)) : 0U;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        f[side_element(p, cell, faces, L3::opp(i))] = encode<T>(p, v[i], i, direction_dither(bits, i));
+        f[side_element(p, cell, faces, L3::opp(i))] = encode<T>(p, v[i], i, direction_dither<T>(bits, i));
     }
 }
 
@@ -512,15 +512,35 @@ __device__ __forceinline__ void load_run(const Lattice3<typename T::R>& p, const
     }
 }
 
+// Direction i of V consecutive cells, whose dither bits are `bits`, as stored. The 16-bit
+// formats round two cells into one 32-bit word (the second conversion writes the word's high
+// half), so that a run's words need no packing.
+template <class T, int V>
+__device__ __forceinline__ void encode_run(const Lattice3<typename T::R>& p, const typename T::R (&in)[V],
+                                           const std::uint32_t (&bits)[V], int i, typename T::S (&out)[V]) {
+    if constexpr (sizeof(typename T::S) == 2 && V % 2 == 0) {
+        std::uint32_t words[V / 2];
+        const typename T::R scaled = storage_offset(p, 19 + i);
+#pragma unroll
+        for (int k = 0; k < V / 2; ++k) {
+            words[k] = T::pack_pair(in[2 * k], in[2 * k + 1], scaled, direction_dither<T>(bits[2 * k], i),
+                                    direction_dither<T>(bits[2 * k + 1], i));
+        }
+        __builtin_memcpy(out, words, sizeof(words));
+    } else {
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            out[k] = encode<T>(p, in[k], i, direction_dither<T>(bits[k], i));
+        }
+    }
+}
+
 // Direction i of V consecutive cells, whose dither bits are `bits`.
 template <class T, int V>
 __device__ __forceinline__ void store_run(const Lattice3<typename T::R>& p, typename T::S* dst, const typename T::R (&in)[V],
                                           const std::uint32_t (&bits)[V], int i) {
     typename T::S tmp[V];
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-        tmp[k] = encode<T>(p, in[k], i, direction_dither(bits[k], i));
-    }
+    encode_run<T, V>(p, in, bits, i, tmp);
     __builtin_memcpy(dst, tmp, sizeof(tmp));
 }
 
@@ -610,10 +630,7 @@ __device__ __forceinline__ void store_shifted(const Lattice3<typename T::R>& p, 
                                               typename T::S* base, std::uint32_t g, bool careful, int i,
                                               const typename T::R (&in)[V], const std::uint32_t (&bits)[V]) {
     typename T::S tmp[V];
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-        tmp[k] = encode<T>(p, in[k], i, direction_dither(bits[k], i));
-    }
+    encode_run<T, V>(p, in, bits, i, tmp);
     if (careful && crosses_block<typename T::R, V>(w, g)) {
 #pragma unroll
         for (int k = 0; k < V; ++k) {
@@ -781,7 +798,7 @@ __device__ __forceinline__ void bulk_run(const Lattice3<typename T::R>& p, std::
                 const std::uint32_t g = g0 + static_cast<std::uint32_t>(c);
 #pragma unroll
                 for (int i = 0; i < 19; ++i) {
-                    const S packed = encode<T>(p, v[c][i], i, direction_dither(bits[c], i));
+                    const S packed = encode<T>(p, v[c][i], i, direction_dither<T>(bits[c], i));
                     if constexpr (kLocal) {
                         *element(base, w.offset(g, static_cast<std::uint32_t>(L3::opp(i)))) = packed;
                     } else {
@@ -963,7 +980,7 @@ __device__ __forceinline__ void boundary_node(const Lattice3<typename T::R>& p, 
     }
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        const S packed = encode<T>(p, v[i], i, direction_dither(bits, i));
+        const S packed = encode<T>(p, v[i], i, direction_dither<T>(bits, i));
         sw[i * bs + b] = decode<T>(p, packed, i);
         if (((wmask >> i) & 1U) == 0U) {
             continue;

@@ -30,7 +30,8 @@ namespace lbm {
 // Each format: S the stored type, R the arithmetic type, `shifted` whether the arithmetic works
 // on f - w. pack(v, dither) rounds a post-collision value for storage with 32 random bits.
 // `offset`: in the D3Q19 lattices the format stores a population's offset from the free-stream
-// equilibrium rather than f - w itself (Lattice3::shift; the D2Q9 kernels store f - w).
+// equilibrium rather than f - w itself (Lattice3::shift; the D2Q9 kernels store f - w), and
+// pack(v, scaled, dither) rounds it.
 //
 // The 16-bit formats round stochastically: the bits below the stored significand are raised by
 // the dither and then cut, an unbiased rounding. Round-to-nearest makes the rounding error a
@@ -57,9 +58,11 @@ __device__ __forceinline__ std::uint32_t cell_dither(std::uint32_t cell, std::ui
 }
 
 // The dither of direction i from its node's bits: the node's word rotated by 7 i (19 distinct
-// rotations), so each direction reads a uniformly distributed window of it.
+// rotations), so each direction reads a uniformly distributed window of it; `T::kDitherRotation`
+// more for a format whose conversion reads its random bits from the top of the word.
+template <class T>
 __device__ __forceinline__ std::uint32_t direction_dither(std::uint32_t bits, int i) {
-    const int r = (7 * i) & 31;
+    const int r = (7 * i + T::kDitherRotation) & 31;
     return r == 0 ? bits : (bits << r) | (bits >> (32 - r));
 }
 
@@ -70,6 +73,7 @@ struct StoreF32 {
     static constexpr bool dithered = false;
     static constexpr bool offset = false;
     static constexpr int id = 0;
+    static constexpr int kDitherRotation = 0;
     static __device__ __forceinline__ R load(S v) { return v; }
     static __device__ __forceinline__ S pack(R v, std::uint32_t) { return v; }
     static __device__ __forceinline__ S nearest(R v) { return v; }
@@ -86,15 +90,10 @@ struct StoreF16 {
     static constexpr bool offset = true;
     static constexpr int id = 1;
     static constexpr float kScale = 32768.0f;
+    // The conversion takes its 13 random bits from bits 19-31 of the word: rotated 19 further,
+    // those are bits 0-12 of the direction's 7 i rotation.
+    static constexpr int kDitherRotation = 19;
     static __device__ __forceinline__ R load(S v) { return static_cast<float>(v) * (1.0f / kScale); }
-    // The 13 bits of the binary32 significand below binary16's are dithered and cut, which
-    // leaves a value binary16 holds exactly (in its normal range; the conversion rounds the
-    // rest to nearest).
-    static __device__ __forceinline__ S pack(R v, std::uint32_t dither) {
-        const std::uint32_t u = __float_as_uint(v * kScale);
-        const std::uint32_t d = (u + (dither & 0x1FFFU)) & ~0x1FFFU;
-        return static_cast<_Float16>(__uint_as_float(d));
-    }
     static __device__ __forceinline__ S nearest(R v) { return static_cast<_Float16>(v * kScale); }
     static __device__ __forceinline__ S from_double(double v) {
         return static_cast<_Float16>(static_cast<float>(v * 32768.0));
@@ -105,10 +104,25 @@ struct StoreF16 {
     static __device__ __forceinline__ R load(S v, R offset) {
         return __builtin_fmaf(static_cast<float>(v), 1.0f / kScale, offset);
     }
+    // Stochastic rounding by the hardware conversion (v_cvt_sr_f16_f32): in binary16's normal
+    // range it adds bits 19-31 of the random word to the 13 binary32 significand bits below
+    // binary16's and cuts them, bit for bit (u + (word >> 19)) & ~0x1FFF as an integer on the
+    // binary32 pattern u, measured over 2.8 M values on gfx950; below that range it rounds at the
+    // subnormal spacing, also stochastically.
     static __device__ __forceinline__ S pack(R v, R scaled, std::uint32_t dither) {
-        const std::uint32_t u = __float_as_uint(__builtin_fmaf(v, kScale, -scaled));
-        const std::uint32_t d = (u + (dither & 0x1FFFU)) & ~0x1FFFU;
-        return static_cast<_Float16>(__uint_as_float(d));
+        using Pair = __fp16 __attribute__((ext_vector_type(2)));
+        const R x = __builtin_fmaf(v, kScale, -scaled);
+        const Pair h = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(Pair, x), x, dither, false);
+        return __builtin_bit_cast(S, h[0]);
+    }
+    // Two values rounded into the low and the high half of one word (the first conversion keeps
+    // the high half of the word it writes, here its own source, which the second replaces).
+    static __device__ __forceinline__ std::uint32_t pack_pair(R v0, R v1, R scaled, std::uint32_t d0, std::uint32_t d1) {
+        using Pair = __fp16 __attribute__((ext_vector_type(2)));
+        const R x0 = __builtin_fmaf(v0, kScale, -scaled);
+        Pair h = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(Pair, x0), x0, d0, false);
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, __builtin_fmaf(v1, kScale, -scaled), d1, true);
+        return __builtin_bit_cast(std::uint32_t, h);
     }
     static constexpr float kOffsetScale = kScale;
 };
@@ -121,12 +135,9 @@ struct StoreBF16 {
     static constexpr bool dithered = true;
     static constexpr bool offset = true;
     static constexpr int id = 2;
+    static constexpr int kDitherRotation = 0;
     static __device__ __forceinline__ R load(S v) {
         return __uint_as_float(static_cast<std::uint32_t>(v) << 16U);
-    }
-    static __device__ __forceinline__ S pack(R v, std::uint32_t dither) {
-        const std::uint32_t u = __float_as_uint(v);
-        return static_cast<S>((u + (dither >> 16U)) >> 16U);
     }
     static __device__ __forceinline__ S nearest(R v) {
         const std::uint32_t u = __float_as_uint(v);
@@ -136,7 +147,25 @@ struct StoreBF16 {
     static __device__ __forceinline__ S from_double(double v) { return nearest(static_cast<float>(v)); }
     // With an offset (no scale: `scaled` is the offset itself).
     static __device__ __forceinline__ R load(S v, R offset) { return load(v) + offset; }
-    static __device__ __forceinline__ S pack(R v, R scaled, std::uint32_t dither) { return pack(v - scaled, dither); }
+    // Stochastic rounding by the hardware conversion (v_cvt_sr_bf16_f32): it adds the top 16
+    // bits of the random word to the 16 binary32 significand bits below bfloat16's and cuts
+    // them, bit for bit (u + (word >> 16)) >> 16 on the binary32 pattern u, measured over 4.2 M
+    // values on gfx950.
+    static __device__ __forceinline__ S pack(R v, R scaled, std::uint32_t dither) {
+        using Pair = __bf16 __attribute__((ext_vector_type(2)));
+        const R x = v - scaled;
+        const Pair b = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(Pair, x), x, dither, false);
+        return __builtin_bit_cast(S, b[0]);
+    }
+    // Two values rounded into the low and the high half of one word (the first conversion keeps
+    // the high half of the word it writes, here its own source, which the second replaces).
+    static __device__ __forceinline__ std::uint32_t pack_pair(R v0, R v1, R scaled, std::uint32_t d0, std::uint32_t d1) {
+        using Pair = __bf16 __attribute__((ext_vector_type(2)));
+        const R x0 = v0 - scaled;
+        Pair b = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(Pair, x0), x0, d0, false);
+        b = __builtin_amdgcn_cvt_sr_bf16_f32(b, v1 - scaled, d1, true);
+        return __builtin_bit_cast(std::uint32_t, b);
+    }
     static constexpr float kOffsetScale = 1.0f;
 };
 
@@ -148,6 +177,7 @@ struct StoreF64 {
     static constexpr bool dithered = false;
     static constexpr bool offset = false;
     static constexpr int id = 3;
+    static constexpr int kDitherRotation = 0;
     static __device__ __forceinline__ R load(S v) { return v; }
     static __device__ __forceinline__ S pack(R v, std::uint32_t) { return v; }
     static __device__ __forceinline__ S nearest(R v) { return v; }
