@@ -115,14 +115,12 @@ inline int default_cells_per_thread(int storage) {
 // 512x256x256 the in-place pattern moves its 19 directions 9% faster in blocks of 2^13 cells
 // than in one plane per direction (its A step 14%, its B step 3%); the bulk kernel's f32 step is
 // fastest at 2^13 (2^12, 2^14, 2^15: 1.5% slower), its f16s step at 2^14-2^15 (1.3% faster than at
-// 2^13). ORRERY_LBM_BLOCK_SHIFT (8 to 16), read from the environment once, forces the block size
-// for layout experiments.
+// 2^13). ORRERY_LBM_BLOCK_SHIFT (8 to 16), read from the environment at each lattice's
+// creation, forces the block size for layout experiments and tests (results do not depend on it).
 inline PopulationLayout population_layout(std::size_t nx, std::size_t ny, std::size_t nz, int storage) {
-    static const std::uint32_t forced = [] {
-        const char* env = std::getenv("ORRERY_LBM_BLOCK_SHIFT");
-        const long v = env != nullptr ? std::strtol(env, nullptr, 10) : 0;
-        return v >= 8 && v <= 16 ? static_cast<std::uint32_t>(v) : 0U;
-    }();
+    const char* env = std::getenv("ORRERY_LBM_BLOCK_SHIFT");
+    const long v = env != nullptr ? std::strtol(env, nullptr, 10) : 0;
+    const std::uint32_t forced = v >= 8 && v <= 16 ? static_cast<std::uint32_t>(v) : 0U;
     PopulationLayout l;
     const std::size_t plane = nx * ny;
     const std::uint32_t most = store_bytes(storage) == 2 ? 14U : 13U;
