@@ -257,11 +257,13 @@ int lbm3d_create(const lbm3d_config* config, const uint8_t* solid, const double*
     }
     h->lds_per_cu = static_cast<std::uint32_t>(lds_cu > 0 ? lds_cu : 0);
     h->lds_per_block = static_cast<std::uint32_t>(lds_block > 0 ? lds_block : 0);
+    // One stream; the second stream of the two-stream schedule is created when that schedule is
+    // selected. An idle second stream is not free: a first lattice of a process whose handle
+    // also held one ran its B steps 4-5% slower (512x256x256 f32 bulk kernel, B step 1027-1030
+    // against 983 us without it), an effect of how the process's streams map to the device's
+    // hardware queues.
     if (e == hipSuccess) {
         e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-    }
-    if (e == hipSuccess) {
-        e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
     }
     if (e == hipSuccess) {
         e = hipEventCreateWithFlags(&h->fork, hipEventDisableTiming);
@@ -857,6 +859,13 @@ int lbm3d_set_schedule(lbm3d* h, int32_t schedule) {
     }
     if (schedule != LBM_SCHEDULE_FUSED && schedule != LBM_SCHEDULE_SERIAL && schedule != LBM_SCHEDULE_STREAMS) {
         return fail(LBM_EINVAL, "lbm3d: unknown step schedule");
+    }
+    if (schedule == LBM_SCHEDULE_STREAMS && h->side == nullptr) {
+        const hipError_t e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            h->side = nullptr;
+            return hip_fail("stream", e);
+        }
     }
     h->schedule = schedule;
     return ok();
