@@ -166,7 +166,8 @@ int lbm3d_set_cells_per_thread(lbm3d* h, int32_t a_step, int32_t b_step);
 int lbm3d_get_cells_per_thread(const lbm3d* h, int32_t* a_step, int32_t* b_step);
 /* How a step is launched: LBM_SCHEDULE_SERIAL (default) runs the bulk kernel then the
  * boundary kernel on the handle's stream; LBM_SCHEDULE_STREAMS runs the two concurrently on
- * two streams; LBM_SCHEDULE_FUSED runs bulk and boundary nodes in one kernel in cell order.
+ * two streams (the second is created when this schedule is first selected); LBM_SCHEDULE_FUSED
+ * runs bulk and boundary nodes in one kernel in cell order.
  * The launch shape of lbm3d_set_launch applies to the bulk kernel of the first two. All three
  * give the same results bit for bit. */
 enum { LBM_SCHEDULE_FUSED = 0, LBM_SCHEDULE_SERIAL = 1, LBM_SCHEDULE_STREAMS = 2 };
