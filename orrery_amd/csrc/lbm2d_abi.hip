@@ -41,6 +41,7 @@ struct lbm2d {
     std::uint32_t* lslot = nullptr;
     double* partial = nullptr;
     double* result = nullptr;
+    void* shift = nullptr;  // the 16-bit formats' storage offsets (Lattice2::shift)
     std::vector<void*> owned;
     // Replayed runs of steps of lbm2d_step.
     lbm::StepGraph graph;
@@ -93,7 +94,29 @@ template <class T> lbm::Lattice2<typename T::R> descriptor2(const lbm2d& h) {
     p.force[1] = static_cast<R>(c.body_force_y);
     p.forced = (c.body_force_x != 0.0 || c.body_force_y != 0.0) ? 1 : 0;
     p.outlet_speed = static_cast<R>(c.inlet_ux);
+    p.shift = static_cast<const R*>(h.shift);
     return p;
+}
+
+// The 16-bit formats' storage offsets: each direction's free-stream equilibrium less its weight,
+// f_eq_i(1, u_ref) - w_i with u_ref the inlet velocity (zero without an inlet), then the same
+// scaled as the format scales its stored values; empty for the other formats.
+template <class T> std::vector<typename T::R> storage_offsets2(const lbm2d_config& c) {
+    using R = typename T::R;
+    std::vector<R> t;
+    if constexpr (T::offset) {
+        t.assign(18, R(0));
+        const bool inflow = c.x_boundary == LBM_X_INFLOW_OUTFLOW;
+        const double ux = inflow ? c.inlet_ux : 0.0;
+        const double uy = inflow ? c.inlet_uy : 0.0;
+        const double uu = ux * ux + uy * uy;
+        for (int i = 0; i < 9; ++i) {
+            const double cu = lbm::D2Q9::cx(i) * ux + lbm::D2Q9::cy(i) * uy;
+            t[i] = static_cast<R>(lbm::D2Q9::w(i) * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu));
+            t[9 + i] = t[i] * T::kOffsetScale;  // exact: a power of two
+        }
+    }
+    return t;
 }
 
 template <class Fn> int with_store2(const lbm2d& h, Fn&& fn) {
@@ -250,6 +273,18 @@ int lbm2d_create(const lbm2d_config* config, const uint8_t* solid, int32_t stora
         (rc = alloc2(*h, &h->lcell, nl * 4U, "link buffer")) || (rc = alloc2(*h, &h->ldir, nl, "link buffer")) ||
         (rc = alloc2(*h, &h->lslot, nl * 4U, "link buffer")) ||
         (rc = alloc2(*h, &h->partial, (red + 8U) * 8U, "reduction buffer"))) {
+        release2(h);
+        return rc;
+    }
+    rc = with_store2(*h, [&]<class T>() -> int {
+        const std::vector<typename T::R> offsets = storage_offsets2<T>(*config);
+        if (offsets.empty()) {
+            return ok();
+        }
+        int r = alloc2(*h, reinterpret_cast<char**>(&h->shift), offsets.size() * sizeof(typename T::R), "storage offsets");
+        return r != LBM_OK ? r : upload2(*h, h->shift, offsets, "offset upload");
+    });
+    if (rc != LBM_OK) {
         release2(h);
         return rc;
     }

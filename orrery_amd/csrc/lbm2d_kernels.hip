@@ -18,6 +18,31 @@ using L2 = D2Q9;
 
 __device__ __forceinline__ int off2(int i, int nx) { return L2::cx(i) + nx * L2::cy(i); }
 
+// Entry i of the storage offset table (read-only while kernels run; read through the constant
+// address space, so uniform reads are scalar loads).
+template <class R> __device__ __forceinline__ R offset2(const Lattice2<R>& p, int i) {
+    using ConstantR = const __attribute__((address_space(4))) R;
+    return ((ConstantR*)p.shift)[i];
+}
+// A stored population of direction i as the arithmetic value, and back: the 16-bit formats
+// store its offset from the free-stream equilibrium (Lattice2::shift), rounded to nearest.
+template <class T>
+__device__ __forceinline__ typename T::R decode2(const Lattice2<typename T::R>& p, typename T::S s, int i) {
+    if constexpr (T::offset) {
+        return T::load(s, offset2(p, i));
+    } else {
+        return T::load(s);
+    }
+}
+template <class T>
+__device__ __forceinline__ typename T::S encode2(const Lattice2<typename T::R>& p, typename T::R v, int i) {
+    if constexpr (T::offset) {
+        return T::nearest(v, offset2(p, 9 + i));
+    } else {
+        return T::nearest(v);
+    }
+}
+
 template <class T>
 __device__ __forceinline__ void collide2(typename T::R (&f)[9], const Lattice2<typename T::R>& p) {
     using R = typename T::R;
@@ -171,18 +196,18 @@ __device__ __forceinline__ void bulk_cell2(const Lattice2<typename T::R>& p, std
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         if constexpr (kLocal) {
-            v[i] = T::load(f[i * stride + cell]);
+            v[i] = decode2<T>(p, f[i * stride + cell], i);
         } else {
-            v[i] = T::load(f[L2::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(cell) - off2(i, nx))]);
+            v[i] = decode2<T>(p, f[L2::opp(i) * stride + static_cast<std::uint32_t>(static_cast<int>(cell) - off2(i, nx))], i);
         }
     }
     collide2<T>(v, p);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         if constexpr (kLocal) {
-            f[L2::opp(i) * stride + cell] = T::nearest(v[i]);
+            f[L2::opp(i) * stride + cell] = encode2<T>(p, v[i], i);
         } else {
-            f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off2(i, nx))] = T::nearest(v[i]);
+            f[i * stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off2(i, nx))] = encode2<T>(p, v[i], i);
         }
     }
 }
@@ -213,9 +238,9 @@ __device__ __forceinline__ void boundary_node2(const Lattice2<typename T::R>& p,
         if (kind == kNode) {
             if ((code & kCodeMirrorY) == 0U) {
                 if constexpr (kLocal) {
-                    v[i] = T::load(f[i * stride + cell]);
+                    v[i] = decode2<T>(p, f[i * stride + cell], i);
                 } else {
-                    v[i] = T::load(f[L2::opp(i) * stride + payload]);
+                    v[i] = decode2<T>(p, f[L2::opp(i) * stride + payload], i);
                 }
             } else {
                 v[i] = st[L2::mirror_y(i) * bs + payload];
@@ -251,8 +276,8 @@ __device__ __forceinline__ void boundary_node2(const Lattice2<typename T::R>& p,
     collide2<T>(v, p);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        const S packed = T::nearest(v[i]);
-        sw[i * bs + b] = T::load(packed);
+        const S packed = encode2<T>(p, v[i], i);
+        sw[i * bs + b] = decode2<T>(p, packed, i);
         if constexpr (kLocal) {
             f[L2::opp(i) * stride + cell] = packed;
         } else {
@@ -345,6 +370,9 @@ __global__ __launch_bounds__(256) void d2q9_init(Lattice2<typename T::R> p, int 
         if constexpr (T::shifted) {
             value -= L2::w(i);
         }
+        if constexpr (T::offset) {
+            value -= static_cast<double>(offset2(p, i));
+        }
         f[L2::opp(i) * p.stride + cell] = T::from_double(value);
     }
 }
@@ -359,7 +387,7 @@ __global__ __launch_bounds__(256) void d2q9_init_boundary(Lattice2<typename T::R
     const std::uint32_t cell = p.bcell[b];
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        const typename T::R v = T::load(f[L2::opp(i) * p.stride + cell]);
+        const typename T::R v = decode2<T>(p, f[L2::opp(i) * p.stride + cell], i);
         p.stash[i * p.bstride + b] = v;
         if (L2::cx(i) < 0) {
             p.hist[L2::outlet_slot(i) * p.bstride + b] = v;
@@ -371,9 +399,9 @@ template <class T>
 __device__ __forceinline__ typename T::R bulk_value2(const Lattice2<typename T::R>& p, const typename T::S* f, int i,
                                                      std::uint32_t cell, int phase) {
     if (phase == 0) {
-        return T::load(f[L2::opp(i) * p.stride + cell]);
+        return decode2<T>(p, f[L2::opp(i) * p.stride + cell], i);
     }
-    return T::load(f[i * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off2(i, static_cast<int>(p.nx)))]);
+    return decode2<T>(p, f[i * p.stride + static_cast<std::uint32_t>(static_cast<int>(cell) + off2(i, static_cast<int>(p.nx)))], i);
 }
 
 template <class T>

@@ -96,6 +96,10 @@ struct StoreF16 {
     static constexpr int kDitherRotation = 19;
     static __device__ __forceinline__ R load(S v) { return static_cast<float>(v) * (1.0f / kScale); }
     static __device__ __forceinline__ S nearest(R v) { return static_cast<_Float16>(v * kScale); }
+    // Round to nearest of the value less an offset, `scaled` = offset 2^15 (the D2Q9 lattices).
+    static __device__ __forceinline__ S nearest(R v, R scaled) {
+        return static_cast<_Float16>(__builtin_fmaf(v, kScale, -scaled));
+    }
     static __device__ __forceinline__ S from_double(double v) {
         return static_cast<_Float16>(static_cast<float>(v * 32768.0));
     }
@@ -144,6 +148,9 @@ struct StoreBF16 {
         const std::uint32_t u = __float_as_uint(v);
         return static_cast<S>((u + 0x7FFFU + ((u >> 16U) & 1U)) >> 16U);
     }
+    // Round to nearest even of the value less an offset (`scaled` is the offset itself; the
+    // D2Q9 lattices).
+    static __device__ __forceinline__ S nearest(R v, R scaled) { return nearest(v - scaled); }
     // Initial states round to nearest even.
     static __device__ __forceinline__ S from_double(double v) { return nearest(static_cast<float>(v)); }
     // With an offset (no scale: `scaled` is the offset itself).
@@ -280,6 +287,7 @@ template <typename R> struct Lattice2 {
     R force[2];
     int forced;
     R outlet_speed;
+    const R* shift;  // formats with `offset`: [18] each direction's offset, then scaled as stored
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X (block b runs on XCD

@@ -114,11 +114,12 @@ def test_one_launch_and_two_launch_steps_are_bitwise_identical(storage, periodic
 
 
 
-# The D2Q9 kernels store the 16-bit formats as f - w rounded to nearest (no free-stream offset,
-# no dither, unlike the D3Q19 ones). Measured on the 400 x 160 cylinder channel (Re 100) after
-# 6000 steps against the f64 storage from the same seeded start: drag mean over steps 3000-6000
-# 1.1e-3 (f16s) and 1.4e-2 (bf16s), velocity L2 7.6e-4 and 7.9e-3; f32 3e-5 and 1.8e-5.
-D2Q9_16_BOUNDS = {"f32": (1e-4, 1e-4), "f16s": (3e-3, 2e-3), "bf16s": (3e-2, 2e-2)}
+# The D2Q9 kernels store the 16-bit formats as the offset from the free-stream equilibrium,
+# rounded to nearest (no dither, unlike the D3Q19 ones). Measured on the 400 x 160 cylinder
+# channel (Re 100) after 6000 steps against the f64 storage from the same seeded start: drag mean
+# over steps 3000-6000 1.0e-3 (f16s) and 8.8e-3 (bf16s), velocity L2 3.4e-4 and 4.5e-3 (f - w
+# stored, without the offset: 1.1e-3 / 1.4e-2 and 7.6e-4 / 7.9e-3); f32 3e-5 and 1.8e-5.
+D2Q9_16_BOUNDS = {"f32": (1e-4, 1e-4), "f16s": (3e-3, 1e-3), "bf16s": (2e-2, 1e-2)}
 
 
 def test_storage_formats_against_f64():
