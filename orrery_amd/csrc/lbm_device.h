@@ -29,10 +29,10 @@ namespace lbm {
 // ---- storage formats ----------------------------------------------------------------------
 // Each format: S the stored type, R the arithmetic type, `shifted` whether the arithmetic works
 // on f - w. The binary32 and binary64 formats store with pack(v, dither) (the dither unused).
-// `offset` (the 16-bit formats): in the D3Q19 lattices the format stores a population's offset
-// from the free-stream equilibrium rather than f - w itself (Lattice3::shift), rounded with 32
-// random bits by pack(v, scaled, dither), or two cells into one word by pack_pair. The D2Q9
-// kernels store f - w in every format, rounded to nearest (nearest).
+// `offset` (the 16-bit formats): the format stores a population's offset from the free-stream
+// equilibrium rather than f - w itself (Lattice3::shift, Lattice2::shift); the D3Q19 kernels
+// round it with 32 random bits by pack(v, scaled, dither), or two cells into one word by
+// pack_pair, the D2Q9 kernels to nearest by nearest(v, scaled).
 //
 // The 16-bit formats round stochastically: the bits below the stored significand are raised by
 // the dither and then cut, an unbiased rounding. Round-to-nearest makes the rounding error a
